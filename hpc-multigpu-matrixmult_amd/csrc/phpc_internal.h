@@ -1,0 +1,83 @@
+// Internal declarations shared by the HIP sources of libphpc (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#include "phpc_gemm.h"
+
+// ---------------------------------------------------------------------------
+// Error handling: every entry point records its status in a thread-local slot
+// (phpc_last_error) and returns it where the signature allows.
+// ---------------------------------------------------------------------------
+extern "C" int phpc_set_error(int code, const char *fmt, ...);
+#define PHPC_HIP(call)                                                                   \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return phpc_set_error(PHPC_ERR_HIP, "%s failed: %s (%s:%d)", #call,                \
+                            hipGetErrorString(e_), __FILE__, __LINE__);                  \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// One GEMM of a grouped launch: C[m x n] (ldc) += A[m x k] (lda) * B[k x n] (ldb),
+// row-major FP64. The k range is walked in panels of `panel` columns; each panel's
+// product is summed from zero and then added to the C value carried from the
+// previous panel, i.e. C = ((C0 + s_0) + s_1) + ... (the grouping produced by one
+// local-GEMM call per SUMMA step, src/phpc_summa.c:63-95 with src/phpc_gemm.cu:32,55).
+// ---------------------------------------------------------------------------
+struct GemmDesc {
+  const double *A;
+  const double *B;
+  double *C;
+  long lda, ldb, ldc;
+  int m, n, k;
+  int panel;       // >= k: one panel
+  int tiles_n;     // tiles along n for this GEMM (filled by the planner)
+  int tile_start;  // first global tile id of this GEMM (prefix sum)
+};
+
+// Kernel-level tile configuration chosen by the planner / autotuner.
+struct F64Config {
+  int id;         // index into the instantiated kernel table
+  int bm, bn;     // macro tile
+  int threads;    // workgroup size
+  int wg_per_cu;  // resident workgroups per CU the config is built for
+};
+
+// Launch policy knobs (see phpc_set_launch_policy in phpc_gemm.h).
+struct LaunchPolicy {
+  int honor_grid;     // 1: persistent grid = caller's grid_width * grid_height
+  int config_id;      // -1: autotuned
+  int grid_override;  // >0: persistent grid size
+};
+LaunchPolicy &phpc_policy();
+
+// Device properties cached per process (one GPU).
+struct DeviceInfo {
+  int device;
+  int cus;
+  int xcds;
+  size_t lds_per_cu;
+};
+const DeviceInfo &phpc_device();
+
+// Grouped FP64 launch. `descs_host` are copied into a library-owned device buffer
+// (stream-ordered). `grid` <= 0 selects the autotuned persistent grid. Returns
+// PHPC_OK or an error code.
+int phpc_f64_grouped(const GemmDesc *descs_host, int ndesc, int grid, int config_id,
+                     hipStream_t stream);
+// Same, but with the descriptor array already resident on the device (graph-friendly).
+int phpc_f64_grouped_dev(const GemmDesc *descs_dev, const GemmDesc *descs_host, int ndesc,
+                         int grid, int config_id, hipStream_t stream);
+int phpc_f64_num_configs();
+F64Config phpc_f64_config(int id);
+int phpc_f64_pick_config(const GemmDesc *descs, int ndesc);
+
+// Strided block pack: dst[r*ldd + c] = src[r*lds + c] for r < rows, c < cols.
+int phpc_pack_launch(const double *src, long lds, double *dst, long ldd, int rows, int cols,
+                     hipStream_t stream);
+
+// Scratch device buffer owned by the library (grows; stream-ordered use only).
+void *phpc_scratch(size_t bytes, int slot);

@@ -1,0 +1,173 @@
+// SUMMA k-panel loop (src/phpc_summa.c:24-122) on an emulated p x q grid, one GPU.
+//
+// Rank (i, j) of the grid owns the C block rows R_i x columns C_j. At step t the owner
+// of A's panel t in grid row i (column t % q) and the owner of B's panel t in grid
+// column j (row t % p) broadcast it (:63-89); every rank then runs the local update
+// C_ij += A(R_i, t) * B(t, C_j) (:93). On one GPU every rank sees the whole of A and B
+// in HBM, so:
+//   FUSED     one persistent grouped launch covers all p*q rank blocks; each tile walks
+//             the panels in order, summing each panel from zero and adding it to the C
+//             value carried in registers, i.e. exactly the sequence of per-step updates;
+//   STEPWISE  per panel, the non-owners' receive buffers are produced by a strided ->
+//             packed copy (the MPI_Type_vector send of :53-56), owners read in place
+//             with ld = N (:72-73, :82-83), then one grouped launch does every rank's
+//             update; the C blocks stay in place (the gather of :97-110 is a no-op);
+//   VENDOR    the same steps with rocBLAS dgemm as the local GEMM (phpc_gemm_cublas).
+// FUSED and STEPWISE run the same per-element arithmetic and give bit-identical C.
+#include "phpc_internal.h"
+#include "phpc_summa.h"
+
+#include <vector>
+
+namespace {
+
+int gcd(int a, int b) {
+  while (b) {
+    int r = a % b;
+    a = b;
+    b = r;
+  }
+  return a;
+}
+
+inline int blk_size(int n, int parts, int i) { return n / parts + (i < n % parts); }
+inline int blk_start(int n, int parts, int i) {
+  const int q = n / parts, r = n % parts;
+  return i * q + (i < r ? i : r);
+}
+
+}  // namespace
+
+extern "C" int phpc_summa_device(int p, int q, int panel, int mode, const double *dA, const double *dB, double *dC,
+                                 int N, int wg, void *stream_) {
+  if (phpc_init()) return phpc_last_error();
+  if (p < 1 || q < 1 || N < 0) return phpc_set_error(PHPC_ERR_ARG, "bad grid %dx%d or N=%d", p, q, N);
+  if (p > N || q > N) return phpc_set_error(PHPC_ERR_ARG, "grid %dx%d larger than N=%d", p, q, N);
+  if (N == 0) return phpc_set_error(PHPC_OK, nullptr);
+  if (!dA || !dB || !dC) return phpc_set_error(PHPC_ERR_ARG, "null matrix");
+  const int lcm = p / gcd(p, q) * q;
+  const int w = panel > 0 ? panel : N / lcm;
+  if (w < 1) return phpc_set_error(PHPC_ERR_ARG, "panel width N/lcm(p,q) = 0 for N=%d, grid %dx%d", N, p, q);
+  hipStream_t s = (hipStream_t)stream_;
+  const LaunchPolicy &pol = phpc_policy();
+  const int G = wg > 0 ? wg : pol.grid_override;
+
+  if (mode == PHPC_SUMMA_FUSED) {
+    std::vector<GemmDesc> d;
+    d.reserve((size_t)p * q);
+    for (int i = 0; i < p; ++i)
+      for (int j = 0; j < q; ++j) {
+        GemmDesc g{};
+        const int r0 = blk_start(N, p, i), c0 = blk_start(N, q, j);
+        g.A = dA + (long)r0 * N;
+        g.B = dB + c0;
+        g.C = dC + (long)r0 * N + c0;
+        g.lda = N;
+        g.ldb = N;
+        g.ldc = N;
+        g.m = blk_size(N, p, i);
+        g.n = blk_size(N, q, j);
+        g.k = N;
+        g.panel = w;
+        d.push_back(g);
+      }
+    int rc = phpc_f64_grouped(d.data(), (int)d.size(), G, pol.config_id, s);
+    if (rc) return rc;
+    return phpc_set_error(PHPC_OK, nullptr);
+  }
+  if (mode != PHPC_SUMMA_STEPWISE && mode != PHPC_SUMMA_VENDOR)
+    return phpc_set_error(PHPC_ERR_ARG, "unknown SUMMA mode %d", mode);
+
+  // receive buffers: one A panel per grid row (m_i x w) and one B panel per grid
+  // column (w x n_j), packed (ld = w and ld = n_j), as src/phpc_summa.c:47-48
+  double *abuf = (double *)phpc_scratch((size_t)N * w * 8, 3);
+  double *bbuf = (double *)phpc_scratch((size_t)N * w * 8, 4);
+  if (!abuf || !bbuf) return phpc_set_error(PHPC_ERR_MEM, "panel buffers");
+  const int steps = (N + w - 1) / w;
+  std::vector<GemmDesc> d((size_t)p * q);
+  for (int t = 0; t < steps; ++t) {
+    const int k0 = t * w;
+    const int kw = k0 + w <= N ? w : N - k0;
+    const int a_owner_col = t % q, b_owner_row = t % p;
+    // "broadcast": non-owner ranks get a packed copy of the panel
+    for (int i = 0; i < p; ++i) {
+      const int r0 = blk_start(N, p, i), mr = blk_size(N, p, i);
+      int rc = phpc_pack_launch(dA + (long)r0 * N + k0, N, abuf + (long)r0 * w, kw, mr, kw, s);
+      if (rc) return rc;
+    }
+    for (int j = 0; j < q; ++j) {
+      const int c0 = blk_start(N, q, j), nc = blk_size(N, q, j);
+      int rc = phpc_pack_launch(dB + (long)k0 * N + c0, N, bbuf + (long)c0 * w, nc, kw, nc, s);
+      if (rc) return rc;
+    }
+    for (int i = 0; i < p; ++i)
+      for (int j = 0; j < q; ++j) {
+        GemmDesc &g = d[(size_t)i * q + j];
+        g = GemmDesc{};
+        const int r0 = blk_start(N, p, i), c0 = blk_start(N, q, j);
+        const int mr = blk_size(N, p, i), nc = blk_size(N, q, j);
+        if (j == a_owner_col) {  // owner: strided block in place, lda = N
+          g.A = dA + (long)r0 * N + k0;
+          g.lda = N;
+        } else {
+          g.A = abuf + (long)r0 * w;
+          g.lda = kw;
+        }
+        if (i == b_owner_row) {
+          g.B = dB + (long)k0 * N + c0;
+          g.ldb = N;
+        } else {
+          g.B = bbuf + (long)c0 * w;
+          g.ldb = nc;
+        }
+        g.C = dC + (long)r0 * N + c0;
+        g.ldc = N;
+        g.m = mr;
+        g.n = nc;
+        g.k = kw;
+        g.panel = 0;
+      }
+    if (mode == PHPC_SUMMA_STEPWISE) {
+      int rc = phpc_f64_grouped(d.data(), (int)d.size(), G, pol.config_id, s);
+      if (rc) return rc;
+    } else {
+      for (const GemmDesc &g : d) {
+        int rc = phpc_dgemm_vendor_device(g.A, g.lda, g.B, g.ldb, g.C, g.ldc, g.m, g.k, g.n, s);
+        if (rc) return rc;
+      }
+    }
+  }
+  return phpc_set_error(PHPC_OK, nullptr);
+}
+
+extern "C" int phpc_summa_host(int p, int q, int panel, int mode, const double *A, const double *B, double *C, int N,
+                               int wg, float *kernel_seconds) {
+  if (kernel_seconds) *kernel_seconds = 0.0f;
+  if (phpc_init()) return phpc_last_error();
+  if (N < 0 || (N > 0 && (!A || !B || !C))) return phpc_set_error(PHPC_ERR_ARG, "bad SUMMA host arguments");
+  if (N == 0) return phpc_set_error(PHPC_OK, nullptr);
+  const size_t bytes = (size_t)N * N * 8;
+  double *dA = (double *)phpc_scratch(bytes, 5);
+  double *dB = (double *)phpc_scratch(bytes, 6);
+  double *dC = (double *)phpc_scratch(bytes, 7);
+  if (!dA || !dB || !dC) return phpc_set_error(PHPC_ERR_MEM, "device allocation failed (3 x %zu bytes)", bytes);
+  hipStream_t s = nullptr;
+  PHPC_HIP(hipMemcpyAsync(dA, A, bytes, hipMemcpyHostToDevice, s));
+  PHPC_HIP(hipMemcpyAsync(dB, B, bytes, hipMemcpyHostToDevice, s));
+  PHPC_HIP(hipMemcpyAsync(dC, C, bytes, hipMemcpyHostToDevice, s));
+  hipEvent_t e0, e1;
+  PHPC_HIP(hipEventCreate(&e0));
+  PHPC_HIP(hipEventCreate(&e1));
+  PHPC_HIP(hipEventRecord(e0, s));
+  int rc = phpc_summa_device(p, q, panel, mode, dA, dB, dC, N, wg, s);
+  PHPC_HIP(hipEventRecord(e1, s));
+  if (rc == PHPC_OK) PHPC_HIP(hipMemcpyAsync(C, dC, bytes, hipMemcpyDeviceToHost, s));
+  PHPC_HIP(hipStreamSynchronize(s));
+  float ms = 0;
+  PHPC_HIP(hipEventElapsedTime(&ms, e0, e1));
+  PHPC_HIP(hipEventDestroy(e0));
+  PHPC_HIP(hipEventDestroy(e1));
+  if (rc) return rc;
+  if (kernel_seconds) *kernel_seconds = ms / 1000.0f;
+  return phpc_set_error(PHPC_OK, nullptr);
+}

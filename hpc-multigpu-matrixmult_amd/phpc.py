@@ -1,0 +1,265 @@
+"""ctypes binding of libphpc's C ABI (include/phpc_gemm.h, include/phpc_summa.h, ...).
+
+This is the harness the tests and the benchmark drive the library through: every call
+goes through the exported C symbols, exactly as a C caller of the reference's
+phpc_gemm.cuh / phpc_summa.h entry points would make it. Host arrays are numpy
+float64 C-contiguous arrays; device buffers are torch CUDA tensors (HIP on ROCm),
+passed as raw pointers.
+
+Loading never falls back to anything else: if lib/libphpc.so is missing the import of
+`lib()` raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parent
+LIB_PATH = HERE / "lib" / "libphpc.so"
+BIN_PATH = HERE / "bin" / "main.out"
+INCLUDE = ROOT / "include"
+
+PHPC_OK = 0
+PHPC_ERR_ARG = 1
+PHPC_ERR_HIP = 2
+PHPC_ERR_NODEV = 3
+PHPC_ERR_MEM = 4
+PHPC_ERR_VENDOR = 5
+PHPC_ERR_BUDGET = 6
+
+SUMMA_FUSED = 0
+SUMMA_STEPWISE = 1
+SUMMA_VENDOR = 2
+
+DIST_UNIFORM = 0
+DIST_WIDE = 1
+DIST_RAMP = 2
+
+_lib = None
+
+c_double_p = ctypes.POINTER(ctypes.c_double)
+c_float_p = ctypes.POINTER(ctypes.c_float)
+c_int_p = ctypes.POINTER(ctypes.c_int)
+vp = ctypes.c_void_p
+L = ctypes.c_long
+I = ctypes.c_int
+
+GEMM_T = ctypes.CFUNCTYPE(None, c_double_p, I, c_double_p, I, c_double_p, I, I, I, I, I, I, I, I, c_float_p)
+
+_SIGS = {
+    # include/phpc_gemm.h
+    "phpc_gemm_cuda": (None, [vp, I, vp, I, vp, I, I, I, I, I, I, I, I, c_float_p]),
+    "phpc_gemm_cublas": (None, [vp, I, vp, I, vp, I, I, I, I, I, I, I, I, c_float_p]),
+    "phpc_last_error": (I, []),
+    "phpc_last_error_message": (ctypes.c_char_p, []),
+    "phpc_init": (I, []),
+    "phpc_dgemm_device": (I, [vp, L, vp, L, vp, L, I, I, I, I, vp]),
+    "phpc_dgemm_vendor_device": (I, [vp, L, vp, L, vp, L, I, I, I, vp]),
+    "phpc_set_launch_policy": (I, [I, I, I]),
+    "phpc_num_configs": (I, []),
+    "phpc_config_info": (I, [I, c_int_p, c_int_p, c_int_p]),
+    "phpc_fill_device": (I, [vp, L, I, I, I, ctypes.c_uint64, vp]),
+    "phpc_check_sampled": (I, [vp, L, vp, L, vp, L, vp, L, I, I, I, c_int_p, c_int_p, I,
+                               c_double_p, c_double_p, vp]),
+    # include/phpc_summa.h
+    "phpc_grid_create": (I, [I, ctypes.POINTER(vp)]),
+    "phpc_grid_create_dims": (I, [I, I, ctypes.POINTER(vp)]),
+    "phpc_grid_destroy": (None, [vp]),
+    "phpc_grid_dims": (I, [vp, c_int_p]),
+    "phpc_grid_set_panel": (I, [vp, I]),
+    "phpc_grid_get_panel": (I, [vp]),
+    "phpc_grid_set_mode": (I, [vp, I]),
+    "find_lcm": (I, [I, I]),
+    "phpc_gemm_summa_cuda": (None, [vp, vp, vp, vp, I, I, I, I, I, c_float_p]),
+    "phpc_gemm_summa_cublas": (None, [vp, vp, vp, vp, I, I, c_float_p]),
+    "phpc_gemm_summa": (I, [GEMM_T, vp, vp, vp, vp, I, I, I, I, I, c_float_p]),
+    "phpc_summa_device": (I, [I, I, I, I, vp, vp, vp, I, I, vp]),
+    "phpc_summa_host": (I, [I, I, I, I, vp, vp, vp, I, I, c_float_p]),
+    # include/utils.h
+    "get_cur_time": (ctypes.c_double, []),
+}
+
+
+def lib():
+    """Load libphpc.so (built by `make -C hpc-multigpu-matrixmult_amd`). Raises if absent."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"libphpc is not built: {LIB_PATH} missing (run __graft_entry__.build())")
+        h = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(h, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _lib = h
+    return _lib
+
+
+def declared_functions():
+    """Every function name declared in include/*.h (the C ABI surface)."""
+    names = set()
+    for h in sorted(INCLUDE.glob("*.h")):
+        text = h.read_text()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = re.sub(r"//[^\n]*", "", text)
+        for m in re.finditer(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(([^;{]*?)\)\s*;", text):
+            name = m.group(1)
+            before = text[: m.start()].rstrip()
+            if name in ("if", "while", "for", "return", "sizeof"):
+                continue
+            # skip function-pointer typedefs: "typedef void (*gemm_t)(...)"
+            if before.endswith("(*") or before.endswith("typedef"):
+                continue
+            names.add(name)
+    names.discard("gemm_t")
+    return names
+
+
+def check(rc: int):
+    if rc != PHPC_OK:
+        msg = lib().phpc_last_error_message()
+        raise RuntimeError(f"libphpc error {rc}: {msg.decode() if msg else ''}")
+
+
+def last_error():
+    l = lib()
+    msg = l.phpc_last_error_message()
+    return l.phpc_last_error(), (msg.decode() if msg else "")
+
+
+# ---------------- host-pointer helpers (numpy) ----------------
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def gemm_cuda(a, b, c, m=None, k=None, n=None, lda=None, ldb=None, ldc=None, gpu_count=1,
+              grid_width=8, grid_height=8, block_width=32):
+    """phpc_gemm_cuda on numpy arrays (C updated in place). Returns kernel seconds."""
+    m = a.shape[0] if m is None else m
+    k = a.shape[1] if k is None else k
+    n = b.shape[1] if n is None else n
+    lda = a.shape[1] if lda is None else lda
+    ldb = b.shape[1] if ldb is None else ldb
+    ldc = c.shape[1] if ldc is None else ldc
+    t = ctypes.c_float(0)
+    lib().phpc_gemm_cuda(_ptr(a), lda, _ptr(b), ldb, _ptr(c), ldc, m, k, n, gpu_count,
+                         grid_width, grid_height, block_width, ctypes.byref(t))
+    return t.value
+
+
+def gemm_cublas(a, b, c, gpu_count=1):
+    m, k = a.shape
+    n = b.shape[1]
+    t = ctypes.c_float(0)
+    lib().phpc_gemm_cublas(_ptr(a), k, _ptr(b), n, _ptr(c), n, m, k, n, gpu_count, 0, 0, 0,
+                           ctypes.byref(t))
+    return t.value
+
+
+class Grid:
+    """Owning wrapper of phpc_grid_t."""
+
+    def __init__(self, nranks=None, dims=None, panel=0, mode=SUMMA_FUSED):
+        h = vp()
+        if dims is not None:
+            check(lib().phpc_grid_create_dims(dims[0], dims[1], ctypes.byref(h)))
+        else:
+            check(lib().phpc_grid_create(nranks, ctypes.byref(h)))
+        self.h = h
+        check(lib().phpc_grid_set_panel(h, panel))
+        check(lib().phpc_grid_set_mode(h, mode))
+
+    def dims(self):
+        d = (ctypes.c_int * 2)()
+        check(lib().phpc_grid_dims(self.h, d))
+        return d[0], d[1]
+
+    def close(self):
+        if self.h:
+            lib().phpc_grid_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def summa_cuda(grid: Grid, A, B, C, gw=8, gh=8, bw=32):
+    t = ctypes.c_float(0)
+    n = A.shape[0]
+    lib().phpc_gemm_summa_cuda(grid.h, _ptr(A), _ptr(B), _ptr(C), n, 1, gw, gh, bw, ctypes.byref(t))
+    return t.value
+
+
+def summa_cublas(grid: Grid, A, B, C):
+    t = ctypes.c_float(0)
+    lib().phpc_gemm_summa_cublas(grid.h, _ptr(A), _ptr(B), _ptr(C), A.shape[0], 1, ctypes.byref(t))
+    return t.value
+
+
+# ---------------- device helpers (torch tensors) ----------------
+
+def dptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def stream_ptr(stream=None):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def fill_device(x, dist, seed, stream=None):
+    m, n = x.shape
+    check(lib().phpc_fill_device(dptr(x), x.stride(0), m, n, dist, seed, stream_ptr(stream)))
+
+
+def dgemm_device(A, B, C, panel=0, stream=None, m=None, k=None, n=None):
+    m = A.shape[0] if m is None else m
+    k = A.shape[1] if k is None else k
+    n = B.shape[1] if n is None else n
+    check(lib().phpc_dgemm_device(dptr(A), A.stride(0), dptr(B), B.stride(0), dptr(C), C.stride(0),
+                                  m, k, n, panel, stream_ptr(stream)))
+
+
+def dgemm_vendor_device(A, B, C, stream=None):
+    m, k = A.shape
+    n = B.shape[1]
+    check(lib().phpc_dgemm_vendor_device(dptr(A), A.stride(0), dptr(B), B.stride(0), dptr(C),
+                                         C.stride(0), m, k, n, stream_ptr(stream)))
+
+
+def summa_device(A, B, C, p, q, panel=0, mode=SUMMA_FUSED, wg=0, stream=None):
+    check(lib().phpc_summa_device(p, q, panel, mode, dptr(A), dptr(B), dptr(C), A.shape[0], wg,
+                                  stream_ptr(stream)))
+
+
+def check_sampled(A, B, C, rows, cols, C0=None, stream=None):
+    import numpy as np
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    cols = np.ascontiguousarray(cols, dtype=np.int32)
+    cw = ctypes.c_double(0)
+    nw = ctypes.c_double(0)
+    c0p = dptr(C0) if C0 is not None else None
+    ldc0 = C0.stride(0) if C0 is not None else 0
+    check(lib().phpc_check_sampled(dptr(A), A.stride(0), dptr(B), B.stride(0), c0p, ldc0, dptr(C),
+                                   C.stride(0), A.shape[0], A.shape[1], B.shape[1],
+                                   rows.ctypes.data_as(c_int_p), cols.ctypes.data_as(c_int_p),
+                                   len(rows), ctypes.byref(cw), ctypes.byref(nw), stream_ptr(stream)))
+    return cw.value, nw.value
+
+
+def set_launch_policy(honor_grid=0, config_id=-1, grid=0):
+    check(lib().phpc_set_launch_policy(honor_grid, config_id, grid))
+
+
+def num_configs():
+    return lib().phpc_num_configs()

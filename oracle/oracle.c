@@ -1,0 +1,257 @@
+/*
+ * CPU oracle for the dense FP64 GEMM hot path. TEST INFRASTRUCTURE ONLY: see
+ * oracle.h. Built with -ffp-contract=off so that every `a * b + c` written here is
+ * rounded twice unless fma() is called explicitly.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define GAMMA 0x9E3779B97F4A7C15ULL
+
+/* ---------------- generators ---------------- */
+
+static inline uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+uint64_t orc_splitmix64_at(uint64_t seed, uint64_t i) { return mix64(seed + (i + 1) * GAMMA); }
+
+void orc_fill_uniform(double *x, size_t n, uint64_t seed) {
+  for (size_t i = 0; i < n; ++i) {
+    uint64_t z = orc_splitmix64_at(seed, i);
+    x[i] = 2.0 * ((double)(z >> 11) * 0x1.0p-53) - 1.0;
+  }
+}
+
+void orc_fill_wide(double *x, size_t n, uint64_t seed) {
+  for (size_t i = 0; i < n; ++i) {
+    uint64_t z = orc_splitmix64_at(seed, i);
+    int e = (int)(((z >> 32) * 41ULL) >> 32) - 20;
+    x[i] = ldexp((z & 1) ? -1.0 : 1.0, e);
+  }
+}
+
+void orc_fill_ramp(double *x, size_t n) {
+  for (size_t i = 0; i < n; ++i) x[i] = (double)i;
+}
+
+/* ---------------- reference arithmetic ---------------- */
+
+/* src/iterative.c:8-13 */
+void orc_iterative(const double *A, const double *B, double *C, int n) {
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      const double a = A[(size_t)i * n + j];
+      const double *brow = B + (size_t)j * n;
+      double *crow = C + (size_t)i * n;
+      for (int k = 0; k < n; ++k) {
+        double p = a * brow[k];
+        crow[k] = crow[k] + p;
+      }
+    }
+}
+
+/* src/phpc_gemm.cu:25-55 (per-element arithmetic of gemm_kernel) */
+void orc_kernel_emul(const double *A, int lda, const double *B, int ldb, double *C, int ldc,
+                     int m, int k, int n) {
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < m; ++i) {
+    double *acc = (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+    const double *arow = A + (size_t)i * lda;
+    /* k outer so that the row of B streams; each acc[j] still sees ascending kk. */
+    for (int kk = 0; kk < k; ++kk) {
+      const double a = arow[kk];
+      const double *brow = B + (size_t)kk * ldb;
+      for (int j = 0; j < n; ++j) acc[j] = fma(a, brow[j], acc[j]);
+    }
+    double *crow = C + (size_t)i * ldc;
+    for (int j = 0; j < n; ++j) crow[j] = crow[j] + acc[j];
+    free(acc);
+  }
+}
+
+/* src/phpc_summa.c:9-22 */
+int orc_find_lcm(int a, int b) {
+  int x = a, y = b;
+  while (y != 0) {
+    int r = x % y;
+    x = y;
+    y = r;
+  }
+  return a / x * b;
+}
+
+/* MPI_Dims_create(P, 2), src/main.c:38-45 */
+void orc_dims_create(int nranks, int dims[2]) {
+  if (nranks <= 1) {
+    dims[0] = dims[1] = 1;
+    return;
+  }
+  int best = 1;
+  for (int d = 1; (long)d * d <= nranks; ++d)
+    if (nranks % d == 0) best = d;
+  dims[0] = nranks / best;
+  dims[1] = best;
+}
+
+int orc_block_size(int n, int parts, int i) { return n / parts + (i < n % parts); }
+
+int orc_block_start(int n, int parts, int i) {
+  int q = n / parts, r = n % parts;
+  return i * q + (i < r ? i : r);
+}
+
+/* src/phpc_summa.c:36-95 on an emulated p x q grid */
+void orc_summa_emul(const double *A, const double *B, double *C, int N, int p, int q, int panel) {
+  int w = panel > 0 ? panel : N / orc_find_lcm(p, q);
+  if (w <= 0) return;
+  int steps = (N + w - 1) / w;
+  for (int t = 0; t < steps; ++t) {
+    int k0 = t * w;
+    int kw = (k0 + w <= N) ? w : N - k0;
+    for (int ri = 0; ri < p; ++ri)
+      for (int cj = 0; cj < q; ++cj) {
+        int r0 = orc_block_start(N, p, ri), mr = orc_block_size(N, p, ri);
+        int c0 = orc_block_start(N, q, cj), nc = orc_block_size(N, q, cj);
+        orc_kernel_emul(A + (size_t)r0 * N + k0, N, B + (size_t)k0 * N + c0, N,
+                        C + (size_t)r0 * N + c0, N, mr, kw, nc);
+      }
+  }
+}
+
+/* ---------------- double-double reference ---------------- */
+
+static inline void two_sum(double a, double b, double *s, double *e) {
+  double x = a + b;
+  double z = x - a;
+  *e = (a - (x - z)) + (b - z);
+  *s = x;
+}
+
+static inline void dd_add_prod(double *hi, double *lo, double a, double b) {
+  double p = a * b;
+  double pe = fma(a, b, -p);
+  double s, se;
+  two_sum(*hi, p, &s, &se);
+  *hi = s;
+  *lo += se + pe;
+}
+
+static double dd_dot(const double *a, size_t inca, const double *b, size_t incb, int k, double c0) {
+  double hi = c0, lo = 0.0;
+  for (int kk = 0; kk < k; ++kk) dd_add_prod(&hi, &lo, a[(size_t)kk * inca], b[(size_t)kk * incb]);
+  return hi + lo;
+}
+
+void orc_gemm_dd(const double *A, int lda, const double *B, int ldb, const double *C0, int ldc0,
+                 double *Cref, int ldcr, int m, int k, int n) {
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int i = 0; i < m; ++i) {
+    double *hi = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    double *lo = (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+    for (int j = 0; j < n; ++j) hi[j] = C0 ? C0[(size_t)i * ldc0 + j] : 0.0;
+    const double *arow = A + (size_t)i * lda;
+    for (int kk = 0; kk < k; ++kk) {
+      const double a = arow[kk];
+      const double *brow = B + (size_t)kk * ldb;
+      for (int j = 0; j < n; ++j) dd_add_prod(&hi[j], &lo[j], a, brow[j]);
+    }
+    for (int j = 0; j < n; ++j) Cref[(size_t)i * ldcr + j] = hi[j] + lo[j];
+    free(hi);
+    free(lo);
+  }
+}
+
+void orc_gemm_dd_entries(const double *A, int lda, const double *B, int ldb, const double *C0,
+                         int ldc0, const int *rows, const int *cols, double *out, size_t count,
+                         int k) {
+#pragma omp parallel for schedule(dynamic, 16)
+  for (long e = 0; e < (long)count; ++e) {
+    int i = rows[e], j = cols[e];
+    double c0 = C0 ? C0[(size_t)i * ldc0 + j] : 0.0;
+    out[e] = dd_dot(A + (size_t)i * lda, 1, B + j, (size_t)ldb, k, c0);
+  }
+}
+
+double orc_dot_q(const double *a, int inca, const double *b, int incb, int k) {
+  __float128 s = 0;
+  for (int kk = 0; kk < k; ++kk) s += (__float128)a[(size_t)kk * inca] * (__float128)b[(size_t)kk * incb];
+  return (double)s;
+}
+
+double orc_absdot(const double *A, int lda, const double *B, int ldb, int i, int j, int k) {
+  double s = 0;
+  for (int kk = 0; kk < k; ++kk) s += fabs(A[(size_t)i * lda + kk]) * fabs(B[(size_t)kk * ldb + j]);
+  return s;
+}
+
+void orc_err_full(const double *C, int ldc, const double *Cref, int ldcr, const double *A, int lda,
+                  const double *B, int ldb, const double *C0, int ldc0, int m, int k, int n,
+                  double *err_cw, double *err_nw) {
+  double cw = 0, maxd = 0, maxr = 0;
+#pragma omp parallel for reduction(max : cw, maxd, maxr) schedule(dynamic, 4)
+  for (int i = 0; i < m; ++i) {
+    double *ab = (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+    for (int kk = 0; kk < k; ++kk) {
+      double a = fabs(A[(size_t)i * lda + kk]);
+      const double *brow = B + (size_t)kk * ldb;
+      for (int j = 0; j < n; ++j) ab[j] += a * fabs(brow[j]);
+    }
+    for (int j = 0; j < n; ++j) {
+      double d = fabs(C[(size_t)i * ldc + j] - Cref[(size_t)i * ldcr + j]);
+      double den = ab[j] + (C0 ? fabs(C0[(size_t)i * ldc0 + j]) : 0.0);
+      double r = d == 0.0 ? 0.0 : (den > 0 ? d / den : INFINITY);
+      if (r > cw) cw = r;
+      if (d > maxd) maxd = d;
+      double ar = fabs(Cref[(size_t)i * ldcr + j]);
+      if (ar > maxr) maxr = ar;
+    }
+    free(ab);
+  }
+  if (err_cw) *err_cw = cw;
+  if (err_nw) *err_nw = maxr > 0 ? maxd / maxr : maxd;
+}
+
+/* ---------------- closed forms and residuals ---------------- */
+
+double orc_ramp_closed(int N, int r, int c) {
+  typedef unsigned __int128 u128;
+  u128 n = (u128)N;
+  u128 s1 = n * (n - 1) / 2;
+  u128 s2 = (n - 1) * n * (2 * n - 1) / 6;
+  u128 v = (u128)r * n * n * s1 + (u128)r * (u128)c * n * n + n * s2 + (u128)c * s1;
+  return (double)v;
+}
+
+double orc_freivalds(const double *A, const double *B, const double *C, int n, uint64_t seed) {
+  double *x = (double *)malloc(sizeof(double) * (size_t)n);
+  double *bx = (double *)malloc(sizeof(double) * (size_t)n);
+  double *absbx = (double *)malloc(sizeof(double) * (size_t)n);
+  orc_fill_uniform(x, (size_t)n, seed);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; ++i) {
+    bx[i] = dd_dot(B + (size_t)i * n, 1, x, 1, n, 0.0);
+    double s = 0;
+    for (int j = 0; j < n; ++j) s += fabs(B[(size_t)i * n + j]) * fabs(x[j]);
+    absbx[i] = s;
+  }
+  double worst = 0;
+#pragma omp parallel for reduction(max : worst) schedule(static)
+  for (int i = 0; i < n; ++i) {
+    double cx = dd_dot(C + (size_t)i * n, 1, x, 1, n, 0.0);
+    double abx = dd_dot(A + (size_t)i * n, 1, bx, 1, n, 0.0);
+    double den = 0;
+    for (int j = 0; j < n; ++j) den += fabs(A[(size_t)i * n + j]) * absbx[j];
+    double r = den > 0 ? fabs(cx - abx) / den : fabs(cx - abx);
+    if (r > worst) worst = r;
+  }
+  free(x);
+  free(bx);
+  free(absbx);
+  return worst;
+}

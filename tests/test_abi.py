@@ -1,0 +1,95 @@
+"""The C-ABI library loads and exports every function declared in include/*.h; the
+host-only C code (grid handle, lcm, the generic SUMMA loop) behaves like the reference
+on the CPU. No GPU compute is called here."""
+import ctypes
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import phpc, ROOT
+import oracle as O
+
+
+def test_library_exports_every_declared_symbol(lib):
+    declared = phpc.declared_functions()
+    assert {"phpc_gemm_cuda", "phpc_gemm_cublas", "phpc_gemm_summa_cuda", "phpc_gemm_summa_cublas",
+            "get_cur_time", "log_to_csv", "find_lcm"} <= declared
+    out = subprocess.run(["nm", "-D", "--defined-only", str(phpc.LIB_PATH)], capture_output=True,
+                         text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    missing = sorted(declared - exported)
+    assert not missing, f"declared but not exported: {missing}"
+
+
+def test_binding_signatures_cover_declarations(lib):
+    # every declared function except log_to_csv (FILE*) has a ctypes signature
+    assert phpc.declared_functions() - set(phpc._SIGS) <= {"log_to_csv"}
+
+
+def test_find_lcm_matches_oracle(lib):
+    for a in range(1, 13):
+        for b in range(1, 13):
+            assert lib.find_lcm(a, b) == O.find_lcm(a, b)
+
+
+@pytest.mark.parametrize("P", [1, 2, 3, 4, 6, 7, 8, 9, 12, 16, 64])
+def test_grid_create_matches_dims_create(lib, P):
+    g = phpc.Grid(nranks=P)
+    assert g.dims() == O.dims_create(P)
+
+
+def test_grid_rejects_bad_arguments(lib):
+    h = ctypes.c_void_p()
+    assert lib.phpc_grid_create(0, ctypes.byref(h)) == phpc.PHPC_ERR_ARG
+    assert lib.phpc_grid_create_dims(2, 0, ctypes.byref(h)) == phpc.PHPC_ERR_ARG
+    g = phpc.Grid(dims=(2, 2))
+    assert lib.phpc_grid_set_panel(g.h, -1) == phpc.PHPC_ERR_ARG
+    assert lib.phpc_grid_set_mode(g.h, 7) == phpc.PHPC_ERR_ARG
+
+
+def test_get_cur_time_is_wall_clock(lib):
+    import time
+    assert abs(lib.get_cur_time() - time.time()) < 1.0
+
+
+def _oracle_gemm_t():
+    """A gemm_t that runs the oracle's emulation of the reference kernel, so the C
+    SUMMA loop can be exercised on the CPU."""
+    calls = []
+
+    def f(a, lda, b, ldb, c, ldc, m, k, n, gpu_count, gw, gh, bw, t):
+        calls.append((m, k, n, lda, ldb, ldc))
+        O.lib().orc_kernel_emul(a, lda, b, ldb, c, ldc, m, k, n)
+        t[0] = 0.5
+
+    return phpc.GEMM_T(f), calls
+
+
+@pytest.mark.parametrize("p,q,N,panel", [(1, 1, 12, 0), (2, 2, 16, 0), (4, 2, 16, 0), (2, 3, 18, 0),
+                                         (3, 3, 20, 4), (4, 4, 24, 5)])
+def test_host_summa_loop_matches_oracle(lib, p, q, N, panel):
+    A, B, C0 = O.uniform((N, N), 1), O.uniform((N, N), 2), O.uniform((N, N), 3)
+    cb, calls = _oracle_gemm_t()
+    g = phpc.Grid(dims=(p, q), panel=panel)
+    C = C0.copy()
+    t = ctypes.c_float(0)
+    rc = lib.phpc_gemm_summa(cb, g.h, phpc._ptr(A), phpc._ptr(B), phpc._ptr(C), N, 1, 8, 8, 32, ctypes.byref(t))
+    assert rc == 0
+    want = O.summa_emul(A, B, C0.copy(), p, q, panel)
+    assert np.array_equal(C.view(np.uint64), want.view(np.uint64))
+    w = panel if panel else N // O.find_lcm(p, q)
+    steps = -(-N // w)
+    assert len(calls) == steps * p * q
+    assert t.value == pytest.approx(0.5 * steps * p * q)
+    # every call sees the full matrices as leading dimension (zero-copy aliases)
+    assert all(c[3] == N and c[4] == N and c[5] == N for c in calls)
+
+
+def test_host_summa_rejects_grid_larger_than_n(lib):
+    cb, _ = _oracle_gemm_t()
+    g = phpc.Grid(dims=(4, 4))
+    A = np.zeros((2, 2))
+    t = ctypes.c_float(0)
+    assert lib.phpc_gemm_summa(cb, g.h, phpc._ptr(A), phpc._ptr(A), phpc._ptr(A), 2, 1, 1, 1, 1,
+                               ctypes.byref(t)) == phpc.PHPC_ERR_ARG

@@ -1,0 +1,117 @@
+"""GPU parity of the emulated SUMMA (src/phpc_summa.c:24-122) against the oracle's
+restatement, bit-exact, plus full-size (N = 16384, 4 x 4 grid, panel 256) properties."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import phpc
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(x):
+    return np.ascontiguousarray(x).view(np.uint64)
+
+
+CASES = [(1, 1, 256, 0), (2, 2, 256, 0), (4, 4, 512, 0), (4, 2, 512, 0), (3, 3, 301, 0),
+         (4, 4, 512, 32), (2, 3, 257, 16), (3, 3, 300, 7)]
+
+
+@pytest.mark.parametrize("p,q,N,panel", CASES)
+def test_summa_device_modes_match_oracle(p, q, N, panel):
+    import torch
+    print(f"summa p={p} q={q} N={N} panel={panel}", flush=True)
+    A, B, C0 = O.uniform((N, N), 1), O.uniform((N, N), 2), O.uniform((N, N), 3)
+    want = O.summa_emul(A, B, C0.copy(), p, q, panel)
+    dA, dB = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
+    for mode in (phpc.SUMMA_FUSED, phpc.SUMMA_STEPWISE):
+        dC = torch.from_numpy(C0.copy()).cuda()
+        phpc.summa_device(dA, dB, dC, p, q, panel=panel, mode=mode)
+        torch.cuda.synchronize()
+        assert np.array_equal(bits(dC.cpu().numpy()), bits(want)), mode
+
+
+@pytest.mark.parametrize("P,N", [(1, 128), (4, 256), (16, 512), (6, 300)])
+def test_summa_cuda_host_entry_point(P, N):
+    A, B, C0 = O.uniform((N, N), 4), O.uniform((N, N), 5), O.uniform((N, N), 6)
+    g = phpc.Grid(nranks=P)
+    p, q = g.dims()
+    C = C0.copy()
+    t = phpc.summa_cuda(g, A, B, C)
+    assert phpc.last_error()[0] == 0 and t > 0
+    assert np.array_equal(bits(C), bits(O.summa_emul(A, B, C0.copy(), p, q, 0)))
+
+
+def test_generic_loop_with_gemm_cuda_equals_device_path(lib):
+    N = 128
+    A, B, C0 = O.uniform((N, N), 7), O.uniform((N, N), 8), O.uniform((N, N), 9)
+    g = phpc.Grid(dims=(2, 4), panel=16)
+    f = phpc.GEMM_T(("phpc_gemm_cuda", lib))
+    C = C0.copy()
+    t = ctypes.c_float(0)
+    assert lib.phpc_gemm_summa(f, g.h, phpc._ptr(A), phpc._ptr(B), phpc._ptr(C), N, 1, 4, 4, 16,
+                               ctypes.byref(t)) == 0
+    assert np.array_equal(bits(C), bits(O.summa_emul(A, B, C0.copy(), 2, 4, 16)))
+    C2 = C0.copy()
+    phpc.summa_cuda(g, A, B, C2)
+    assert np.array_equal(bits(C2), bits(C))
+
+
+def test_summa_ramp_closed_form():
+    N = 1024
+    A = O.ramp((N, N))
+    g = phpc.Grid(nranks=16, panel=0)
+    C = np.zeros((N, N))
+    phpc.summa_cuda(g, A, A, C)
+    assert np.array_equal(C, O.ramp_closed_matrix(N))
+
+
+def test_summa_wide_exponent_bitwise():
+    import torch
+    N = 512
+    A, B = O.wide((N, N), 31), O.wide((N, N), 32)
+    want = O.summa_emul(A, B, np.zeros((N, N)), 4, 4, 64)
+    dC = torch.zeros((N, N), dtype=torch.float64, device="cuda")
+    phpc.summa_device(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(), dC, 4, 4, panel=64)
+    torch.cuda.synchronize()
+    assert np.array_equal(bits(dC.cpu().numpy()), bits(want))
+
+
+def test_summa_cublas_within_bound():
+    N = 256
+    A, B = O.uniform((N, N), 1), O.uniform((N, N), 2)
+    g = phpc.Grid(nranks=4)
+    C = np.zeros((N, N))
+    t = phpc.summa_cublas(g, A, B, C)
+    assert phpc.last_error()[0] == 0 and t == 0.0
+    cw, _ = O.err_full(C, O.gemm_dd(A, B), A, B)
+    assert cw <= 1e-14
+
+
+@pytest.mark.parametrize("dist", [phpc.DIST_UNIFORM, phpc.DIST_WIDE])
+def test_summa_full_size_properties(dist):
+    """N = 16384 on the emulated 4 x 4 grid with panel 256: the fused and stepwise
+    schedules agree bit for bit over the whole matrix, and sampled entries are within
+    the FP64 bound of the double-double value."""
+    import torch
+    N = 16384
+    print(f"summa full size dist={dist}", flush=True)
+    dA = torch.empty((N, N), dtype=torch.float64, device="cuda")
+    dB = torch.empty_like(dA)
+    phpc.fill_device(dA, dist, 1)
+    phpc.fill_device(dB, dist, 2)
+    dC1 = torch.zeros_like(dA)
+    phpc.summa_device(dA, dB, dC1, 4, 4, panel=256, mode=phpc.SUMMA_FUSED)
+    dC2 = torch.zeros_like(dA)
+    phpc.summa_device(dA, dB, dC2, 4, 4, panel=256, mode=phpc.SUMMA_STEPWISE)
+    torch.cuda.synchronize()
+    assert torch.equal(dC1.view(torch.int64), dC2.view(torch.int64))
+    del dC2
+    rng = np.random.default_rng(3)
+    rows = np.concatenate([rng.integers(0, N, 2048), np.arange(0, N, 64)])
+    cols = np.concatenate([rng.integers(0, N, 2048), np.arange(N - 1, -1, -64)])
+    cw, nw = phpc.check_sampled(dA, dB, dC1, rows, cols)
+    print(f"  err_cw={cw:.3e} err_nw={nw:.3e}", flush=True)
+    assert cw <= 1e-14

@@ -1,0 +1,252 @@
+"""Pins of the CPU oracle (oracle/oracle.c) against values it does not produce itself:
+the reference's own iterative.c (built from its sources into oracle/_ref), published
+SplitMix64 outputs, closed forms, brute-force integers, hand-computed known answers,
+binary128 and exact rational arithmetic."""
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+GOLDEN = __import__("pathlib").Path(__file__).resolve().parent / "golden"
+
+
+# ---------------- generators ----------------
+
+def test_splitmix64_published_sequence():
+    # SplitMix64 seeded with 0 (Steele, Lea, Flood 2014; the reference implementation
+    # by Vigna): first outputs of next().
+    want = [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F,
+            0xF88BB8A8724C81EC, 0x1B39896A51A8749B]
+    assert [O.splitmix64_at(0, i) for i in range(5)] == want
+
+
+def test_uniform_range_and_moments():
+    x = O.uniform((200000,), 11)
+    assert x.min() >= -1.0 and x.max() < 1.0
+    assert abs(x.mean()) < 0.01 and abs(x.var() - 1 / 3) < 0.01
+    # the value is 2*(top 53 bits)*2^-53 - 1: every sample is a multiple of 2^-52
+    assert np.all(np.ldexp(x + 1.0, 52) == np.floor(np.ldexp(x + 1.0, 52)))
+
+
+def test_wide_exponents():
+    x = O.wide((100000,), 5)
+    m, e = np.frexp(np.abs(x))
+    assert np.all(m == 0.5)
+    ex = e - 1
+    assert ex.min() == -20 and ex.max() == 20
+    assert len(np.unique(ex)) == 41
+    assert abs((x > 0).mean() - 0.5) < 0.01
+
+
+def test_ramp():
+    assert np.array_equal(O.ramp((3, 3)).ravel(), np.arange(9.0))
+
+
+# ---------------- iterative.c ----------------
+
+@pytest.mark.skipif(not O.ref_iterative_available(), reason="oracle/_ref not built (reference sources absent)")
+@pytest.mark.parametrize("n,dist", [(1, "u"), (17, "u"), (64, "u"), (48, "w"), (40, "r")])
+def test_iterative_matches_reference_build(n, dist):
+    if dist == "u":
+        A, B = O.uniform((n, n), 3), O.uniform((n, n), 4)
+    elif dist == "w":
+        A, B = O.wide((n, n), 3), O.wide((n, n), 4)
+    else:
+        A, B = O.ramp((n, n)), O.ramp((n, n))
+    C0 = O.uniform((n, n), 9)
+    mine = O.iterative(A, B, C0.copy())
+    ref = O.ref_iterative(A, B, C0.copy())
+    assert np.array_equal(mine.view(np.uint64), ref.view(np.uint64))
+
+
+def test_iterative_matches_golden_from_reference():
+    z = np.load(GOLDEN / "ref_iterative_n48.npz")
+    C = O.iterative(z["A"], z["B"], np.zeros_like(z["C"]))
+    assert np.array_equal(C.view(np.uint64), z["C"].view(np.uint64))
+
+
+# ---------------- closed forms and brute force ----------------
+
+@pytest.mark.parametrize("N", [1, 2, 3, 7, 16])
+def test_ramp_closed_form_vs_integers(N):
+    A = np.arange(N * N, dtype=object).reshape(N, N)
+    exact = A.dot(A)
+    for r in range(N):
+        for c in range(N):
+            assert O.ramp_closed(N, r, c) == float(exact[r, c])
+
+
+@pytest.mark.parametrize("N", [5, 64, 130])
+def test_ramp_exact_paths(N):
+    A = O.ramp((N, N))
+    want = O.ramp_closed_matrix(N)
+    assert np.array_equal(O.iterative(A, A, np.zeros((N, N))), want)
+    assert np.array_equal(O.kernel_emul(A, A, np.zeros((N, N))), want)
+    assert np.array_equal(O.summa_emul(A, A, np.zeros((N, N)), 2, 3, 7), want)
+    assert np.array_equal(O.gemm_dd(A, A), want)
+
+
+def test_ramp_golden():
+    z = np.load(GOLDEN / "ramp_n64.npz")
+    assert np.array_equal(O.ramp_closed_matrix(64), z["C"])
+
+
+def test_kernel_emul_integer_brute_force():
+    rng = np.random.default_rng(1)
+    for m, k, n in [(1, 1, 1), (5, 3, 7), (33, 65, 17), (64, 64, 64)]:
+        A = rng.integers(-8, 9, (m, k))
+        B = rng.integers(-8, 9, (k, n))
+        C0 = rng.integers(-100, 101, (m, n))
+        want = (A @ B + C0).astype(np.float64)
+        got = O.kernel_emul(A.astype(np.float64), B.astype(np.float64), C0.astype(np.float64))
+        assert np.array_equal(got, want)
+
+
+def test_kernel_emul_leading_dimensions():
+    # operands embedded in larger arrays: only the m x k / k x n / m x n windows are used
+    rng = np.random.default_rng(2)
+    A = rng.integers(-4, 5, (6, 11)).astype(np.float64)
+    B = rng.integers(-4, 5, (9, 13)).astype(np.float64)
+    C = np.full((6, 10), 7.0)
+    O.kernel_emul(A, B, C, m=5, k=9, n=8, lda=11, ldb=13, ldc=10)
+    want = np.full((6, 10), 7.0)
+    want[:5, :8] += A[:5, :9] @ B[:9, :8]
+    assert np.array_equal(C, want)
+
+
+# ---------------- hand-computed known answers ----------------
+
+def test_fma_chain_vs_unfused_known_answer():
+    # a0*b0 = 1 + 2^-29 + 2^-60 rounds to 1 + 2^-29; a1*b1 is its exact negative.
+    # kernel (fma chain): fma(a1, b1, 1 + 2^-29) = -2^-60 exactly.
+    # iterative (round each product, then add): 0.    exact value: 0.
+    e = 1.0 + 2.0 ** -30
+    A = np.array([[e, e], [0.0, 0.0]])
+    B = np.array([[e, 0.0], [-e, 0.0]])
+    assert O.kernel_emul(A, B, np.zeros((2, 2)))[0, 0] == -(2.0 ** -60)
+    assert O.iterative(A, B, np.zeros((2, 2)))[0, 0] == 0.0
+    assert O.gemm_dd(A, B)[0, 0] == 0.0
+
+
+def test_summa_panel_grouping_known_answer():
+    # C0 = 1, panel products s0 = s1 = 2^-53.
+    # SUMMA with panel 1: (1 + 2^-53) + 2^-53 -> each add ties to even -> 1.
+    # one panel: s = fma(2^-53, 1, 2^-53) = 2^-52, then 1 + 2^-52.
+    t = 2.0 ** -53
+    A = np.array([[t, t], [0.0, 0.0]])
+    B = np.array([[1.0, 0.0], [1.0, 0.0]])
+    C0 = np.array([[1.0, 0.0], [0.0, 0.0]])
+    assert O.summa_emul(A, B, C0.copy(), 1, 1, panel=1)[0, 0] == 1.0
+    assert O.summa_emul(A, B, C0.copy(), 1, 1, panel=2)[0, 0] == 1.0 + 2.0 ** -52
+    # the reference default panel N / lcm(p, q) with a 2 x 2 grid on N = 2 is 1
+    assert O.summa_emul(A, B, C0.copy(), 2, 2, panel=0)[0, 0] == 1.0
+
+
+def test_summa_single_panel_equals_kernel():
+    A, B, C0 = O.uniform((40, 40), 1), O.uniform((40, 40), 2), O.uniform((40, 40), 3)
+    a = O.summa_emul(A, B, C0.copy(), 1, 1, panel=0)
+    b = O.kernel_emul(A, B, C0.copy())
+    assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+def test_summa_independent_of_grid_for_fixed_panel():
+    # the grid only partitions C; the arithmetic per element depends on the panel width
+    A, B = O.uniform((30, 30), 4), O.uniform((30, 30), 5)
+    ref = O.summa_emul(A, B, np.zeros((30, 30)), 1, 1, panel=4)
+    for p, q in [(2, 2), (3, 2), (5, 3), (4, 4)]:
+        got = O.summa_emul(A, B, np.zeros((30, 30)), p, q, panel=4)
+        assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
+
+
+# ---------------- double-double vs independent arithmetic ----------------
+
+def test_dd_cancellation_known_answer():
+    A = np.array([[1e16, 1.0, -1e16]])
+    B = np.array([[1.0], [1.0], [1.0]])
+    assert O.gemm_dd(A, B)[0, 0] == 1.0
+
+
+def test_dd_vs_exact_rationals():
+    rng = np.random.default_rng(5)
+    for trial in range(20):
+        k = 60
+        a = rng.standard_normal(k) * np.exp2(rng.integers(-30, 30, k))
+        b = rng.standard_normal(k) * np.exp2(rng.integers(-30, 30, k))
+        exact = float(sum(Fraction(x) * Fraction(y) for x, y in zip(a, b)))
+        got = O.gemm_dd(a.reshape(1, k), b.reshape(k, 1))[0, 0]
+        assert abs(got - exact) <= math.ulp(exact), (trial, got, exact)
+        assert abs(O.dot_q(a, b) - exact) <= math.ulp(exact)
+
+
+def test_dd_entries_match_full():
+    A, B, C0 = O.uniform((20, 50), 1), O.uniform((50, 30), 2), O.uniform((20, 30), 3)
+    full = O.gemm_dd(A, B, C0)
+    rows = np.array([0, 5, 19, 7])
+    cols = np.array([0, 29, 3, 11])
+    assert np.array_equal(O.gemm_dd_entries(A, B, rows, cols, C0), full[rows, cols])
+
+
+def test_kernel_emul_within_fp64_bound():
+    A, B = O.uniform((37, 300), 8), O.uniform((300, 29), 9)
+    C = O.kernel_emul(A, B, np.zeros((37, 29)))
+    ref = O.gemm_dd(A, B)
+    cw, nw = O.err_full(C, ref, A, B)
+    assert 0 < cw <= 300 * 2.0 ** -53
+
+
+# ---------------- grid helpers ----------------
+
+def test_find_lcm():
+    for a in range(1, 30):
+        for b in range(1, 30):
+            assert O.find_lcm(a, b) == a * b // math.gcd(a, b)
+
+
+def test_dims_create_known_answers_and_brute_force():
+    # MPI standard examples for 2 dimensions: (6) -> (3,2), (7) -> (7,1)
+    assert O.dims_create(6) == (3, 2)
+    assert O.dims_create(7) == (7, 1)
+    assert O.dims_create(16) == (4, 4)
+    assert O.dims_create(8) == (4, 2)
+    assert O.dims_create(1) == (1, 1)
+    for P in range(1, 200):
+        d0, d1 = O.dims_create(P)
+        pairs = [(P // d, d) for d in range(1, P + 1) if P % d == 0 and P // d >= d]
+        best = min(pairs, key=lambda t: t[0] - t[1])
+        assert (d0, d1) == best
+
+
+def test_block_split():
+    assert [O.block_size(10000, 3, i) for i in range(3)] == [3334, 3333, 3333]
+    for n in (7, 100, 10000):
+        for parts in (1, 2, 3, 4, 7):
+            sizes = [O.block_size(n, parts, i) for i in range(parts)]
+            starts = [O.block_start(n, parts, i) for i in range(parts)]
+            assert sum(sizes) == n and max(sizes) - min(sizes) <= 1
+            assert starts == [sum(sizes[:i]) for i in range(parts)]
+
+
+# ---------------- residual and error metrics ----------------
+
+def test_freivalds_detects_planted_error():
+    n = 96
+    A, B = O.uniform((n, n), 1), O.uniform((n, n), 2)
+    C = O.gemm_dd(A, B)
+    assert O.freivalds(A, B, C) < 1e-15
+    # a single wrong entry moves row 3 of C x by 1e-6 * |x_5|, far above rounding level
+    C[3, 5] += 1e-6
+    assert O.freivalds(A, B, C) > 1e-11
+
+
+def test_err_full_planted():
+    A, B = O.uniform((8, 8), 1), O.uniform((8, 8), 2)
+    ref = O.gemm_dd(A, B)
+    C = ref.copy()
+    C[2, 3] += 1e-10
+    cw, nw = O.err_full(C, ref, A, B)
+    ab = np.abs(A) @ np.abs(B)
+    assert cw == pytest.approx(1e-10 / ab[2, 3], rel=1e-6)
+    assert nw == pytest.approx(1e-10 / np.abs(ref).max(), rel=1e-6)
