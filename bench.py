@@ -112,35 +112,38 @@ def main():
     phpc.fill_device(A, phpc.DIST_UNIFORM, 1)
     phpc.fill_device(B, phpc.DIST_UNIFORM, 2)
 
-    def step():
-        rc = lib.phpc_summa_device(p, q, panel, phpc.SUMMA_FUSED, phpc.dptr(A), phpc.dptr(B), phpc.dptr(C), N, 0, sp)
+    def run_mode(mode):
+        rc = lib.phpc_summa_device(p, q, panel, mode, phpc.dptr(A), phpc.dptr(B), phpc.dptr(C), N, 0, sp)
         if rc:
             phpc.check(rc)
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
+    def timed(mode, steps, warmup):
+        for _ in range(warmup):
+            run_mode(mode)
+        torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        t_start = torch.cuda.Event(enable_timing=True)
+        t_end = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        w0 = time.perf_counter()
+        t_start.record(stream)
+        for i in range(steps):
+            ev[i][0].record(stream)
+            run_mode(mode)
+            ev[i][1].record(stream)
+        t_end.record(stream)
+        torch.cuda.synchronize()
+        wall_s = time.perf_counter() - w0
+        return t_start.elapsed_time(t_end) / steps, statistics.mean(a.elapsed_time(b) for a, b in ev), wall_s
 
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
-    t_start = torch.cuda.Event(enable_timing=True)
-    t_end = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    wall0 = time.perf_counter()
-    t_start.record(stream)
-    for i in range(args.steps):
-        ev[i][0].record(stream)
-        step()
-        ev[i][1].record(stream)
-    t_end.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - wall0
-    total_ms = t_start.elapsed_time(t_end)
-    launch_ms = [a.elapsed_time(b) for a, b in ev]
-    ms_per_step = total_ms / args.steps
     flops = 2.0 * N ** 3
+    # primary: CHAINED schedule (one FMA chain per output over the 64 panels)
+    ms_per_step, kernel_ms, wall = timed(phpc.SUMMA_CHAINED, args.steps, args.warmup)
     value = flops / (ms_per_step * 1e-3) / 1e12
-    kernel_ms = statistics.mean(launch_ms)
     achieved = flops / (kernel_ms * 1e-3) / 1e12
+    # the reference's per-panel grouping (C rounded after every panel), same launch shape
+    exact_ms, _, _ = timed(phpc.SUMMA_FUSED, max(2, args.steps // 2), 1)
+    step = lambda: run_mode(phpc.SUMMA_CHAINED)  # noqa: E731
 
     # accuracy of the timed path: one clean pass, sampled double-double check on the device
     C.zero_()
@@ -194,7 +197,8 @@ def main():
             "N": N,
             "grid": f"{p}x{q}",
             "panel": panel,
-            "path": "plain FP64 MFMA, fused SUMMA (one persistent launch per pass)",
+            "path": "plain FP64 MFMA, SUMMA CHAINED schedule: one persistent launch per pass, "
+                    "256x128 tiles, one FMA chain per output over the 64 panels",
             "flops_per_step": flops,
         },
         "accuracy": {"err_cw_max_sampled": err_cw, "err_nw_sampled": err_nw, "samples": 4096,
@@ -206,10 +210,11 @@ def main():
             "unit": "TFLOP/s",
             "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
             "traffic": traffic_from_profiles(workload),
-            "kernel": "f64_gemm_kernel<128,128,2,4,MP=1,VEC=2>",
+            "kernel": "f64_gemm_kernel<256,128,4,2,BK=16,MP=0,VEC=2>",
             "kernel_ms_avg": round(kernel_ms, 3),
         },
         "wall_s_timed": round(wall, 3),
+        "summa_panel_exact_tflops": round(flops / (exact_ms * 1e-3) / 1e12, 3),
     }
     out.update(extra)
     if not args.no_cpu:

@@ -35,12 +35,48 @@ struct GemmBatch {
   int ndesc;
   int total_tiles;
   int group_m;            // tile-row swizzle width
+  int xcd_remap;          // 1: contiguous tile runs per XCD
   GemmDesc d0;            // inline copy when ndesc == 1
 };
 
-constexpr int BK = 16;
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+typedef double __attribute__((address_space(1))) gdouble;
 
-template <int BM, int BN, int WM, int WN, bool MP, int VEC>
+// Out-of-range offset: raw buffer loads past num_records return zero, which is the
+// zero padding of partial tiles (rows >= m, columns >= n, k beyond the panel).
+constexpr uint32_t kOOB = 0x80000000u;
+
+__device__ __forceinline__ rsrc_t make_rsrc(const double *base, long bytes) {
+  const uint32_t nb = bytes > 0x7fffffffL ? 0x7fffffffu : (uint32_t)(bytes < 0 ? 0 : bytes);
+  // 0x00020000: gfx9-family buffer descriptor word 3 (32-bit data format, raw access)
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(base), (short)0, (int)nb, 0x00020000);
+}
+
+template <int VEC>
+struct Chunk;
+template <>
+struct Chunk<1> {
+  double x;
+};
+template <>
+struct Chunk<2> {
+  double x, y;
+};
+
+template <int VEC>
+__device__ __forceinline__ Chunk<VEC> buf_load(rsrc_t r, uint32_t off) {
+  Chunk<VEC> c;
+  if constexpr (VEC == 2) {
+    const double2 v = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+    c.x = v.x;
+    c.y = v.y;
+  } else {
+    c.x = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+  }
+  return c;
+}
+
+template <int BM, int BN, int WM, int WN, int BK, bool MP, int VEC>
 struct F64Gemm {
   static constexpr int NT = WM * WN * 64;
   static constexpr int WTM = BM / WM;
@@ -58,16 +94,18 @@ struct F64Gemm {
   static_assert(TM >= 1 && TN >= 1, "wave tile must hold at least one 16x16 block");
 };
 
-template <int VEC>
-struct Chunk;
-template <>
-struct Chunk<1> {
-  double x;
-};
-template <>
-struct Chunk<2> {
-  double x, y;
-};
+// Values that are the same for every lane of the workgroup (tile descriptors), moved to
+// scalar registers so that buffer descriptors built from them need no per-lane loop.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ long uni(long v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+  return (long)(((uint64_t)hi << 32) | lo);
+}
+template <typename T>
+__device__ __forceinline__ T *uni(T *p) {
+  return reinterpret_cast<T *>(uni((long)reinterpret_cast<uintptr_t>(p)));
+}
 
 // Decode global tile id -> (descriptor, tile row, tile col).
 __device__ __forceinline__ const GemmDesc *find_desc(const GemmBatch &b, int t) {
@@ -83,9 +121,29 @@ __device__ __forceinline__ const GemmDesc *find_desc(const GemmBatch &b, int t) 
   return &b.descs[lo];
 }
 
-template <int BM, int BN, int WM, int WN, bool MP, int VEC>
+// Descriptor of tile t with every field moved to scalar registers (the tile id and
+// hence the descriptor are the same for the whole workgroup).
+__device__ __forceinline__ GemmDesc uniform_desc(const GemmBatch &b, int t) {
+  const GemmDesc &d = *find_desc(b, t);
+  GemmDesc u;
+  u.A = uni(d.A);
+  u.B = uni(d.B);
+  u.C = uni(d.C);
+  u.lda = uni(d.lda);
+  u.ldb = uni(d.ldb);
+  u.ldc = uni(d.ldc);
+  u.m = uni(d.m);
+  u.n = uni(d.n);
+  u.k = uni(d.k);
+  u.panel = uni(d.panel);
+  u.tiles_n = uni(d.tiles_n);
+  u.tile_start = uni(d.tile_start);
+  return u;
+}
+
+template <int BM, int BN, int WM, int WN, int BK, bool MP, int VEC>
 __global__ __launch_bounds__(WM *WN * 64) void f64_gemm_kernel(GemmBatch batch) {
-  using K = F64Gemm<BM, BN, WM, WN, MP, VEC>;
+  using K = F64Gemm<BM, BN, WM, WN, BK, MP, VEC>;
   __shared__ double lds[2 * K::STAGE];
 
   const int tid = threadIdx.x;
@@ -100,13 +158,13 @@ __global__ __launch_bounds__(WM *WN * 64) void f64_gemm_kernel(GemmBatch batch) 
   const int bid = blockIdx.x;
   // XCD-aware order: workgroup bid runs on XCD bid % 8; give each XCD a contiguous
   // run of the tiles processed in the same round so neighbouring tiles share its L2.
-  const bool xcd_remap = (G % 8) == 0;
-  const int slot = xcd_remap ? (bid % 8) * (G / 8) + bid / 8 : bid;
+  const bool xcd_remap = batch.xcd_remap && (G % 8) == 0;
+  const int slot = uni(xcd_remap ? (bid % 8) * (G / 8) + bid / 8 : bid);
 
   for (int round = 0;; ++round) {
     const int t = round * G + slot;
     if (t >= batch.total_tiles) break;
-    const GemmDesc &d = *find_desc(batch, t);
+    const GemmDesc d = uniform_desc(batch, uni(t));
     const int lt = t - d.tile_start;
     const int tiles_n = d.tiles_n;
     const int tiles_m = (d.m + BM - 1) / BM;
@@ -120,13 +178,16 @@ __global__ __launch_bounds__(WM *WN * 64) void f64_gemm_kernel(GemmBatch batch) 
     const int tn_idx = in_grp / gsz;
     const int row0 = tm_idx * BM;
     const int col0 = tn_idx * BN;
-    const int m_rem = d.m - row0;
-    const int n_rem = d.n - col0;
-    const double *__restrict__ Ag = d.A + (long)row0 * d.lda;
-    const double *__restrict__ Bg = d.B + col0;
-    const long lda = d.lda, ldb = d.ldb;
-    const int Kdim = d.k;
-    const int w = d.panel > 0 && d.panel < Kdim ? d.panel : Kdim;
+    const int m_rem = uni(d.m - row0);
+    const int n_rem = uni(d.n - col0);
+    const long lda = uni(d.lda), ldb = uni(d.ldb), ldc = uni(d.ldc);
+    const double *Ag = uni(d.A) + (long)row0 * lda;
+    const double *Bg = uni(d.B) + col0;
+    gdouble *Cg = (gdouble *)(uni(d.C) + (long)row0 * ldc + col0);
+    const rsrc_t rA = make_rsrc(Ag, (long)min(m_rem, BM) * lda * 8);
+    const int Kdim = uni(d.k);
+    const int dpanel = uni(d.panel);
+    const int w = dpanel > 0 && dpanel < Kdim ? dpanel : Kdim;
 
     d4 acc[K::TM][K::TN];
     d4 creg[MP ? K::TM : 1][MP ? K::TN : 1];
@@ -144,7 +205,7 @@ __global__ __launch_bounds__(WM *WN * 64) void f64_gemm_kernel(GemmBatch batch) 
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int rr = wm * K::WTM + i * 16 + 4 * r + g;
-            creg[i][j][r] = (rr < m_rem && cc < n_rem) ? d.C[(long)(row0 + rr) * d.ldc + col0 + cc] : 0.0;
+            creg[i][j][r] = (rr < m_rem && cc < n_rem) ? Cg[(long)rr * ldc + cc] : 0.0;
           }
         }
     }
@@ -159,38 +220,17 @@ __global__ __launch_bounds__(WM *WN * 64) void f64_gemm_kernel(GemmBatch batch) 
         const int q = tid + i * K::NT;
         const int row = q / (BK / VEC);
         const int kc = (q % (BK / VEC)) * VEC;
-        const double *p = Ag + (long)row * lda + kb + kc;
-        if constexpr (VEC == 2) {
-          if (row < m_rem && kc + 1 < krem) {
-            const double2 v = *reinterpret_cast<const double2 *>(p);
-            ra[i].x = v.x;
-            ra[i].y = v.y;
-          } else {
-            ra[i].x = (row < m_rem && kc < krem) ? p[0] : 0.0;
-            ra[i].y = 0.0;
-          }
-        } else {
-          ra[i].x = (row < m_rem && kc < krem) ? p[0] : 0.0;
-        }
+        const uint32_t off = (row < m_rem && kc < krem) ? (uint32_t)(((long)row * lda + kb + kc) * 8) : kOOB;
+        ra[i] = buf_load<VEC>(rA, off);
       }
+      const rsrc_t rB = make_rsrc(Bg + (long)kb * ldb, (long)krem * ldb * 8);
 #pragma unroll
       for (int i = 0; i < K::CB; ++i) {
         const int q = tid + i * K::NT;
         const int kr = q / (BN / VEC);
         const int cc = (q % (BN / VEC)) * VEC;
-        const double *p = Bg + (long)(kb + kr) * ldb + cc;
-        if constexpr (VEC == 2) {
-          if (kr < krem && cc + 1 < n_rem) {
-            const double2 v = *reinterpret_cast<const double2 *>(p);
-            rb[i].x = v.x;
-            rb[i].y = v.y;
-          } else {
-            rb[i].x = (kr < krem && cc < n_rem) ? p[0] : 0.0;
-            rb[i].y = 0.0;
-          }
-        } else {
-          rb[i].x = (kr < krem && cc < n_rem) ? p[0] : 0.0;
-        }
+        const uint32_t off = (kr < krem && cc < n_rem) ? (uint32_t)(((long)kr * ldb + cc) * 8) : kOOB;
+        rb[i] = buf_load<VEC>(rB, off);
       }
     };
 
@@ -218,24 +258,27 @@ __global__ __launch_bounds__(WM *WN * 64) void f64_gemm_kernel(GemmBatch batch) 
       }
     };
 
+    auto mfma_step = [&](const double *At, const double *Bs, int kk) {
+      double a[K::TM], b[K::TN];
+#pragma unroll
+      for (int i = 0; i < K::TM; ++i) a[i] = At[(kk * 4 + g) * K::APITCH + wm * K::WTM + i * 16 + li];
+#pragma unroll
+      for (int j = 0; j < K::TN; ++j) b[j] = Bs[(kk * 4 + g) * K::BPITCH + wn * K::WTN + j * 16 + li];
+#pragma unroll
+      for (int i = 0; i < K::TM; ++i)
+#pragma unroll
+        for (int j = 0; j < K::TN; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+    };
+
     auto compute = [&](int buf, int klen) {
       const double *At = lds + buf * K::STAGE;
       const double *Bs = At + BK * K::APITCH;
+      // k beyond the slab was loaded as zero, so a partial slab adds fma(0, 0, c) = c
+      // terms only (the reference kernel pads its own T-wide phases the same way)
+      (void)klen;
 #pragma unroll
-      for (int kk = 0; kk < BK / 4; ++kk) {
-        if (kk * 4 < klen) {
-          double a[K::TM], b[K::TN];
-#pragma unroll
-          for (int i = 0; i < K::TM; ++i) a[i] = At[(kk * 4 + g) * K::APITCH + wm * K::WTM + i * 16 + li];
-#pragma unroll
-          for (int j = 0; j < K::TN; ++j) b[j] = Bs[(kk * 4 + g) * K::BPITCH + wn * K::WTN + j * 16 + li];
-#pragma unroll
-          for (int i = 0; i < K::TM; ++i)
-#pragma unroll
-            for (int j = 0; j < K::TN; ++j)
-              acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
-        }
-      }
+      for (int kk = 0; kk < BK / 4; ++kk) mfma_step(At, Bs, kk);
     };
 
     // All waves must be done with the previous tile's last slab before the prologue
@@ -284,7 +327,7 @@ __global__ __launch_bounds__(WM *WN * 64) void f64_gemm_kernel(GemmBatch batch) 
         for (int r = 0; r < 4; ++r) {
           const int rr = wm * K::WTM + i * 16 + 4 * r + g;
           if (rr < m_rem && cc < n_rem) {
-            double *pc = d.C + (long)(row0 + rr) * d.ldc + col0 + cc;
+            gdouble *pc = Cg + (long)rr * ldc + cc;
             if constexpr (MP)
               *pc = creg[i][j][r];
             else
@@ -295,58 +338,358 @@ __global__ __launch_bounds__(WM *WN * 64) void f64_gemm_kernel(GemmBatch batch) 
   }
 }
 
-typedef void (*KernelFn)(GemmBatch);
+// ---------------------------------------------------------------------------
+// Direct-to-LDS variant (16-byte aligned operands, even k and n).
+//
+// Slabs are copied HBM -> LDS by buffer_load_dwordx4 ... lds: each wave instruction
+// fills one contiguous 1 KiB LDS block, lane p writing bytes [16p, 16p+16). Which
+// global 16-byte chunk each lane fetches is free, so the lane -> address map sets
+// the LDS layout. No registers stage the data, so STAGES slabs can be in flight.
+//
+// LDS layout of one stage, in 16-byte chunks:
+//   A: BM rows x 16 doubles. Rows go in pairs of 17 chunks (8 + 8 + 1 pad), row r at
+//      chunk (r/2)*17 + (r%2)*8. A 16-lane MFMA operand read (16 rows, one k) then
+//      touches 16 distinct 8-byte bank slots, and k steps are plain address offsets.
+//   B: 16 rows x BN doubles, row k at chunk k*BN/2, chunk c at slot c ^ ((k&1)<<3), so
+//      the two 16-lane groups of a half-wave (rows k, k+1) use opposite bank halves.
+// Blocks of a stage are dealt to waves round-robin; pad chunks are fetched with an
+// out-of-range offset (zero) and never read.
+// ---------------------------------------------------------------------------
+typedef __attribute__((address_space(3))) void lds_void;
 
-struct ConfigEntry {
-  int bm, bn, threads;
-  KernelFn fn[2][2];  // [multi-panel][vec2]
+// s_waitcnt vmcnt(N) leaving expcnt and lgkmcnt unconstrained (gfx9 encoding: vmcnt
+// bits [3:0] and [15:14], expcnt [6:4], lgkmcnt [11:8])
+template <int N>
+__device__ __forceinline__ void wait_vm_c() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt(0x0F70 | (N & 0xF) | (((N >> 4) & 3) << 14));
+}
+
+template <int BM, int BN_, int WM, int WN, int STAGES, bool MP>
+struct F64Lds {
+  static constexpr int BK = 16;
+  static constexpr int BN = BN_;
+  static constexpr int NT = WM * WN * 64;
+  static constexpr int NW = WM * WN;
+  static constexpr int WTM = BM / WM;
+  static constexpr int WTN = BN_ / WN;
+  static constexpr int TM = WTM / 16;
+  static constexpr int TN = WTN / 16;
+  static constexpr int A_CHUNKS = (BM / 2) * 17;
+  static constexpr int A_BLOCKS = (A_CHUNKS + 63) / 64;
+  static constexpr int A_BYTES = A_BLOCKS * 1024;
+  static constexpr int B_BLOCKS = BK * BN_ * 8 / 1024;
+  static constexpr int B_BYTES = B_BLOCKS * 1024;
+  static constexpr int STAGE_BYTES = A_BYTES + B_BYTES;
+  static constexpr int BLOCKS = A_BLOCKS + B_BLOCKS;
+  static constexpr int PER_LO = BLOCKS / NW;                  // blocks per wave (low)
+  static constexpr int PER_HI = (BLOCKS + NW - 1) / NW;       // blocks per wave (high)
+  static_assert(BN_ % 32 == 0 && BM % 32 == 0, "tile shape");
+  static_assert(STAGES >= 2 && STAGES <= 4, "pipeline depth");
+  static_assert(STAGES * STAGE_BYTES <= 160 * 1024, "LDS budget");
 };
 
-#define PHPC_F64_CFG(BM_, BN_, WM_, WN_)                                                     \
-  ConfigEntry {                                                                            \
-    BM_, BN_, WM_ *WN_ * 64, {                                                             \
-      {f64_gemm_kernel<BM_, BN_, WM_, WN_, false, 1>, f64_gemm_kernel<BM_, BN_, WM_, WN_, false, 2>}, \
-      {f64_gemm_kernel<BM_, BN_, WM_, WN_, true, 1>, f64_gemm_kernel<BM_, BN_, WM_, WN_, true, 2>}    \
-    }                                                                                      \
+// wait until at most `n` vector-memory operations of this thread are outstanding,
+// n = later * per with per in {PER_LO, PER_HI} and later <= STAGES-2
+template <int LO, int HI>
+__device__ __forceinline__ void wait_outstanding(int later, int per) {
+  const int n = later * per;
+  if (n <= 0) { wait_vm_c<0>(); return; }
+  if (n == LO) { wait_vm_c<LO>(); return; }
+  if (n == HI) { wait_vm_c<HI>(); return; }
+  if (n == 2 * LO) { wait_vm_c<2 * LO>(); return; }
+  if (n == 2 * HI) { wait_vm_c<2 * HI>(); return; }
+  wait_vm_c<0>();
+}
+
+template <int BM, int BN, int WM, int WN, int STAGES, bool MP>
+__global__ __launch_bounds__(WM *WN * 64) void f64_lds_kernel(GemmBatch batch) {
+  using K = F64Lds<BM, BN, WM, WN, STAGES, MP>;
+  extern __shared__ __attribute__((aligned(1024))) char smem[];
+
+  const int tid = threadIdx.x;
+  const int wave = uni(tid >> 6);  // wave-uniform: keeps LDS block bases scalar
+  const int lane = tid & 63;
+  const int g = lane >> 4;
+  const int li = lane & 15;
+  const int wm = wave / WN;
+  const int wn = wave % WN;
+  // blocks owned by this wave: wave, wave + NW, ... (uniform per wave)
+  const int my_blocks = (K::BLOCKS - wave + K::NW - 1) / K::NW;
+
+  // per-lane LDS read offsets (bytes, stage-relative); k steps add constants
+  int a_off[K::TM], b_off[K::TN];
+#pragma unroll
+  for (int i = 0; i < K::TM; ++i) {
+    const int r = wm * K::WTM + i * 16 + li;
+    a_off[i] = ((r >> 1) * 17 + (r & 1) * 8) * 16 + g * 8;
+  }
+#pragma unroll
+  for (int j = 0; j < K::TN; ++j) {
+    const int col = wn * K::WTN + j * 16 + li;
+    b_off[j] = K::A_BYTES + g * K::BN * 8 + ((((col >> 1) ^ ((g & 1) << 3))) << 4) + (col & 1) * 8;
   }
 
+  const int G = gridDim.x;
+  const int bid = blockIdx.x;
+  const bool xcd_remap = batch.xcd_remap && (G % 8) == 0;
+  const int slot = uni(xcd_remap ? (bid % 8) * (G / 8) + bid / 8 : bid);
+
+  for (int round = 0;; ++round) {
+    const int t = round * G + slot;
+    if (t >= batch.total_tiles) break;
+    const GemmDesc d = uniform_desc(batch, uni(t));
+    const int lt = t - d.tile_start;
+    const int tiles_n = d.tiles_n;
+    const int tiles_m = (d.m + BM - 1) / BM;
+    const int gm = batch.group_m;
+    const int grp = lt / (gm * tiles_n);
+    const int first_m = grp * gm;
+    const int gsz = min(tiles_m - first_m, gm);
+    const int in_grp = lt - grp * gm * tiles_n;
+    const int row0 = (first_m + in_grp % gsz) * BM;
+    const int col0 = (in_grp / gsz) * BN;
+    const int m_rem = uni(d.m - row0);
+    const int n_rem = uni(d.n - col0);
+    const long lda = uni(d.lda), ldb = uni(d.ldb), ldc = uni(d.ldc);
+    const double *Ag = uni(d.A) + (long)row0 * lda;
+    const double *Bg = uni(d.B) + col0;
+    gdouble *Cg = (gdouble *)(uni(d.C) + (long)row0 * ldc + col0);
+    const int Kdim = uni(d.k);
+    const int dpanel = uni(d.panel);
+    const int w = dpanel > 0 && dpanel < Kdim ? dpanel : Kdim;
+    const rsrc_t rA = make_rsrc(Ag, (long)min(m_rem, BM) * lda * 8);
+
+    // Per-lane global offsets of this wave's blocks for a full slab starting at k = 0
+    // (A: add kb*8 through soffset; B: the descriptor base moves to row kb). For a
+    // partial slab the k bound is applied on top (kcol < krem).
+    uint32_t goff[K::PER_HI];
+    int gk[K::PER_HI];  // k column (A) or k row (B) of the chunk; -1 for pad chunks
+#pragma unroll
+    for (int u = 0; u < K::PER_HI; ++u) {
+      const int blk = wave + u * K::NW;
+      goff[u] = kOOB;
+      gk[u] = 1 << 30;
+      if (blk < K::A_BLOCKS) {
+        const int q = blk * 64 + lane;  // chunk slot in the A region
+        const int pr = q / 17, pc = q % 17;
+        const int row = 2 * pr + (pc >= 8 ? 1 : 0);
+        const int c = pc >= 8 ? pc - 8 : pc;
+        if (pc < 16 && q < K::A_CHUNKS && row < m_rem) {
+          goff[u] = (uint32_t)(((long)row * lda + 2 * c) * 8);
+          gk[u] = 2 * c;
+        }
+      } else if (blk < K::BLOCKS) {
+        const int q = (blk - K::A_BLOCKS) * 64 + lane;
+        const int kr = q / (BN / 2);
+        const int c = (q % (BN / 2)) ^ ((kr & 1) << 3);
+        if (2 * c < n_rem) {
+          goff[u] = (uint32_t)(((long)kr * ldb + 2 * c) * 8);
+          gk[u] = kr;
+        }
+      }
+    }
+
+    d4 acc[K::TM][K::TN];
+    d4 creg[MP ? K::TM : 1][MP ? K::TN : 1];
+#pragma unroll
+    for (int i = 0; i < K::TM; ++i)
+#pragma unroll
+      for (int j = 0; j < K::TN; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+    if constexpr (MP) {
+#pragma unroll
+      for (int i = 0; i < K::TM; ++i)
+#pragma unroll
+        for (int j = 0; j < K::TN; ++j) {
+          const int cc = wn * K::WTN + j * 16 + li;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int rr = wm * K::WTM + i * 16 + 4 * r + g;
+            creg[i][j][r] = (rr < m_rem && cc < n_rem) ? Cg[(long)rr * ldc + cc] : 0.0;
+          }
+        }
+    }
+
+    const int spp = (w + K::BK - 1) / K::BK;  // slabs per full panel
+    const int full_panels = Kdim / w;
+    const int tail = Kdim - full_panels * w;
+    const int nslab = full_panels * spp + (tail + K::BK - 1) / K::BK;
+
+    auto issue = [&](int sidx) {
+      const int p = sidx / spp;
+      const int kb = p * w + (sidx - p * spp) * K::BK;
+      const int krem = min(min(kb + K::BK, p * w + w), Kdim) - kb;
+      char *stage = smem + (sidx % STAGES) * K::STAGE_BYTES;
+      const rsrc_t rB = make_rsrc(Bg + (long)kb * ldb, (long)krem * ldb * 8);
+#pragma unroll
+      for (int u = 0; u < K::PER_HI; ++u) {
+        const int blk = wave + u * K::NW;
+        if (u < K::PER_LO || blk < K::BLOCKS) {
+          const uint32_t off = gk[u] < krem ? goff[u] : kOOB;
+          if (blk < K::A_BLOCKS)
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rA, (lds_void *)(stage + blk * 1024), 16, off, kb * 8, 0, 0);
+          else
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rB, (lds_void *)(stage + blk * 1024), 16, off, 0, 0, 0);
+        }
+      }
+    };
+
+    auto mfma_step = [&](const char *stage, int kk) {
+      double a[K::TM], b[K::TN];
+#pragma unroll
+      for (int i = 0; i < K::TM; ++i) a[i] = *reinterpret_cast<const double *>(stage + a_off[i] + kk * 32);
+#pragma unroll
+      for (int j = 0; j < K::TN; ++j)
+        b[j] = *reinterpret_cast<const double *>(stage + b_off[j] + kk * 4 * K::BN * 8);
+#pragma unroll
+      for (int i = 0; i < K::TM; ++i)
+#pragma unroll
+        for (int j = 0; j < K::TN; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+    };
+
+    // every wave must be done reading the previous tile's stages before refilling them
+    __syncthreads();
+#pragma unroll
+    for (int s0 = 0; s0 < STAGES - 1; ++s0)
+      if (s0 < nslab) issue(s0);
+
+    for (int sidx = 0; sidx < nslab; ++sidx) {
+      // loads of slab sidx are complete once at most the later slabs' are in flight
+      wait_outstanding<K::PER_LO, K::PER_HI>(min(STAGES - 2, nslab - 1 - sidx), my_blocks);
+      __syncthreads();
+      if (sidx + STAGES - 1 < nslab) issue(sidx + STAGES - 1);
+      const int p = sidx / spp;
+      const int j = sidx - p * spp;
+      const int kb = p * w + j * K::BK;
+      const int pe = min(p * w + w, Kdim);
+      const int klen = min(kb + K::BK, pe) - kb;
+      const char *stage = smem + (sidx % STAGES) * K::STAGE_BYTES;
+      // k beyond the slab reads as zero (out-of-range loads), so a partial slab simply
+      // adds fma(0, 0, c) = c terms, like the reference kernel's own tile padding
+#pragma unroll
+      for (int kk = 0; kk < K::BK / 4; ++kk) mfma_step(stage, kk);
+      if constexpr (MP) {
+        if (kb + klen == pe) {
+#pragma unroll
+          for (int i = 0; i < K::TM; ++i)
+#pragma unroll
+            for (int jj = 0; jj < K::TN; ++jj) {
+              creg[i][jj] = creg[i][jj] + acc[i][jj];
+              acc[i][jj] = d4{0.0, 0.0, 0.0, 0.0};
+            }
+        }
+      }
+    }
+
+#pragma unroll
+    for (int i = 0; i < K::TM; ++i)
+#pragma unroll
+      for (int jj = 0; jj < K::TN; ++jj) {
+        const int cc = wn * K::WTN + jj * 16 + li;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int rr = wm * K::WTM + i * 16 + 4 * r + g;
+          if (rr < m_rem && cc < n_rem) {
+            gdouble *pc = Cg + (long)rr * ldc + cc;
+            if constexpr (MP)
+              *pc = creg[i][jj][r];
+            else
+              *pc = *pc + acc[i][jj][r];
+          }
+        }
+      }
+  }
+}
+
+typedef void (*KernelFn)(GemmBatch);
+
+// One tile configuration: kernels for [panel grouping][16-byte loads]. Direct-to-LDS
+// configs use f64_lds_kernel when 16-byte loads are legal and fall back to the
+// register-staged kernel with the same macro tile otherwise (same tiles, same
+// arithmetic, so the planner's tile list is valid for both).
+struct ConfigEntry {
+  int bm, bn, threads, bk;
+  size_t lds_bytes[2];  // dynamic LDS for [vec1][vec2] kernels
+  bool panels;          // panel-grouped variants fit without spills
+  KernelFn fn[2][2];
+};
+
+#define PHPC_V1(BM_, BN_, WM_, WN_, BK_, MP_, VEC_) f64_gemm_kernel<BM_, BN_, WM_, WN_, BK_, MP_, VEC_>
+#define PHPC_LDS(BM_, BN_, WM_, WN_, ST_, MP_) f64_lds_kernel<BM_, BN_, WM_, WN_, ST_, MP_>
+#define LDS_BYTES(BM_, BN_, ST_) ((size_t)(ST_) * (size_t)F64Lds<BM_, BN_, 2, 2, ST_, false>::STAGE_BYTES)
+
 const ConfigEntry kConfigs[] = {
-    PHPC_F64_CFG(128, 128, 2, 4),  // 0: 8 waves, wave tile 64x32
-    PHPC_F64_CFG(128, 128, 2, 2),  // 1: 4 waves, wave tile 64x64
-    PHPC_F64_CFG(128, 64, 2, 2),   // 2: 4 waves, wave tile 64x32
-    PHPC_F64_CFG(64, 64, 2, 2),    // 3: 4 waves, wave tile 32x32
-    PHPC_F64_CFG(256, 128, 4, 2),  // 4: 8 waves, wave tile 64x64
+    // 0: register-staged, 8 waves (wave tile 64x32), 16-deep slabs
+    {128, 128, 512, 16, {0, 0}, true,
+     {{PHPC_V1(128, 128, 2, 4, 16, false, 1), PHPC_V1(128, 128, 2, 4, 16, false, 2)},
+      {PHPC_V1(128, 128, 2, 4, 16, true, 1), PHPC_V1(128, 128, 2, 4, 16, true, 2)}}},
+    // 1: register-staged, 8 waves, 32-deep slabs
+    {128, 128, 512, 32, {0, 0}, true,
+     {{PHPC_V1(128, 128, 2, 4, 32, false, 1), PHPC_V1(128, 128, 2, 4, 32, false, 2)},
+      {PHPC_V1(128, 128, 2, 4, 32, true, 1), PHPC_V1(128, 128, 2, 4, 32, true, 2)}}},
+    // 2: register-staged, 4 waves (wave tile 32x32), 64x64 tiles for small problems
+    {64, 64, 256, 16, {0, 0}, true,
+     {{PHPC_V1(64, 64, 2, 2, 16, false, 1), PHPC_V1(64, 64, 2, 2, 16, false, 2)},
+      {PHPC_V1(64, 64, 2, 2, 16, true, 1), PHPC_V1(64, 64, 2, 2, 16, true, 2)}}},
+    // 3: direct-to-LDS, 8 waves (wave tile 64x32), 4-stage pipeline
+    {128, 128, 512, 16, {0, LDS_BYTES(128, 128, 4)}, true,
+     {{PHPC_V1(128, 128, 2, 4, 16, false, 1), PHPC_LDS(128, 128, 2, 4, 4, false)},
+      {PHPC_V1(128, 128, 2, 4, 16, true, 1), PHPC_LDS(128, 128, 2, 4, 4, true)}}},
+    // 4: direct-to-LDS, 4 waves (wave tile 64x64), 4-stage pipeline; single panel only
+    {128, 128, 256, 16, {0, LDS_BYTES(128, 128, 4)}, false,
+     {{PHPC_V1(128, 128, 2, 2, 16, false, 1), PHPC_LDS(128, 128, 2, 2, 4, false)},
+      {PHPC_V1(128, 128, 2, 4, 16, true, 1), PHPC_LDS(128, 128, 2, 2, 4, true)}}},
+    // 5: direct-to-LDS, 4 waves (wave tile 32x32), 64x64 tiles, 4-stage pipeline
+    {64, 64, 256, 16, {0, LDS_BYTES(64, 64, 4)}, true,
+     {{PHPC_V1(64, 64, 2, 2, 16, false, 1), PHPC_LDS(64, 64, 2, 2, 4, false)},
+      {PHPC_V1(64, 64, 2, 2, 16, true, 1), PHPC_LDS(64, 64, 2, 2, 4, true)}}},
+    // 6: register-staged, 8 waves (wave tile 64x64), 256x128 tiles; single panel only
+    {256, 128, 512, 16, {0, 0}, false,
+     {{PHPC_V1(256, 128, 4, 2, 16, false, 1), PHPC_V1(256, 128, 4, 2, 16, false, 2)},
+      {PHPC_V1(256, 128, 4, 2, 16, false, 1), PHPC_V1(256, 128, 4, 2, 16, false, 2)}}},
+    // 7: direct-to-LDS, 8 waves (wave tile 64x64), 256x128 tiles, 3 stages; single panel only
+    {256, 128, 512, 16, {0, LDS_BYTES(256, 128, 3)}, false,
+     {{PHPC_V1(256, 128, 4, 2, 16, false, 1), PHPC_LDS(256, 128, 4, 2, 3, false)},
+      {PHPC_V1(256, 128, 4, 2, 16, false, 1), PHPC_LDS(256, 128, 4, 2, 3, false)}}},
 };
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 
-struct OccKey {
-  int occ[kNumConfigs][2][2];
-  bool ready = false;
-};
-
 std::mutex g_mu;
-OccKey g_occ;
+bool g_occ_ready = false;
+int g_occ[kNumConfigs][2][2];
 GemmDesc *g_desc_dev = nullptr;
 size_t g_desc_cap = 0;
 GemmDesc *g_desc_pinned = nullptr;
 size_t g_pinned_cap = 0;
 
+// Note: config 4's multi-panel VEC1 fallback uses the 8-wave kernel, whose launch
+// shape differs; launch() below takes the thread count from the kernel actually used.
+int kernel_threads(int cfg, int mp, int vec2) {
+  if (cfg == 4 && mp && !vec2) return 512;
+  return kConfigs[cfg].threads;
+}
+
+void init_kernels_locked() {
+  if (g_occ_ready) return;
+  for (int c = 0; c < kNumConfigs; ++c)
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 2; ++b) {
+        const void *fn = reinterpret_cast<const void *>(kConfigs[c].fn[a][b]);
+        const size_t lds = kConfigs[c].lds_bytes[b];
+        if (lds > 0) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, kernel_threads(c, a, b), lds) != hipSuccess || n < 1)
+          n = 1;
+        g_occ[c][a][b] = n;
+      }
+  g_occ_ready = true;
+}
+
 int occupancy(int cfg, int mp, int vec2) {
   std::lock_guard<std::mutex> lk(g_mu);
-  if (!g_occ.ready) {
-    for (int c = 0; c < kNumConfigs; ++c)
-      for (int a = 0; a < 2; ++a)
-        for (int b = 0; b < 2; ++b) {
-          int n = 0;
-          if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void *>(kConfigs[c].fn[a][b]),
-                                                           kConfigs[c].threads, 0) != hipSuccess ||
-              n < 1)
-            n = 1;
-          g_occ.occ[c][a][b] = n;
-        }
-    g_occ.ready = true;
-  }
-  return g_occ.occ[cfg][mp][vec2];
+  init_kernels_locked();
+  return g_occ[cfg][mp][vec2];
 }
 
 bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
@@ -366,19 +709,26 @@ F64Config phpc_f64_config(int id) {
   return c;
 }
 
-// Planner: large problems take the 128x128 8-wave tile; small ones a tile that still
-// yields enough workgroups to cover the 256 CUs.
+// Planner. Single-panel work on enough 256x128 tiles to cover the CUs takes the
+// 256x128 tile (16 independent MFMA chains per wave); panel-grouped work takes the
+// 128x128 tile, whose waves also carry C in registers; small problems take 64x64.
+int g_big_config = 0;       // large, panel-grouped
+int g_big_single = 6;       // large, one panel
+int g_group_m = 8;
+int g_xcd_remap = 1;
+
 int phpc_f64_pick_config(const GemmDesc *descs, int ndesc) {
-  long work_tiles128 = 0;
-  for (int i = 0; i < ndesc; ++i)
-    work_tiles128 += (long)((descs[i].m + 127) / 128) * ((descs[i].n + 127) / 128);
+  long tiles128 = 0, tiles256 = 0;
+  bool mp = false;
+  for (int i = 0; i < ndesc; ++i) {
+    tiles128 += (long)((descs[i].m + 127) / 128) * ((descs[i].n + 127) / 128);
+    tiles256 += (long)((descs[i].m + 255) / 256) * ((descs[i].n + 127) / 128);
+    if (descs[i].panel > 0 && descs[i].panel < descs[i].k) mp = true;
+  }
   const int cus = phpc_device().cus;
-  if (work_tiles128 >= 2L * cus) return 0;
-  long work_tiles64 = 0;
-  for (int i = 0; i < ndesc; ++i)
-    work_tiles64 += (long)((descs[i].m + 127) / 128) * ((descs[i].n + 63) / 64);
-  if (work_tiles64 >= 2L * cus) return 2;
-  return 3;
+  if (!mp && tiles256 >= (long)cus) return g_big_single;
+  if (tiles128 >= (long)cus) return g_big_config;
+  return 5;
 }
 
 int phpc_f64_grouped_dev(const GemmDesc *descs_dev, const GemmDesc *descs_host, int ndesc, int grid,
@@ -392,18 +742,25 @@ int phpc_f64_grouped_dev(const GemmDesc *descs_dev, const GemmDesc *descs_host, 
   for (int i = 0; i < ndesc; ++i) {
     const GemmDesc &d = descs_host[i];
     if (d.panel > 0 && d.panel < d.k) mp = true;
-    const bool a_ok = aligned16(d.A) && (d.lda % 2 == 0) &&
+    // 32-bit buffer offsets: one A tile (bm rows) and one B slab (bk rows) must span < 2 GiB
+    if ((long)ce.bm * d.lda * 8 >= 0x7fffffffL || (long)ce.bk * d.ldb * 8 >= 0x7fffffffL)
+      return phpc_set_error(PHPC_ERR_ARG, "leading dimension too large (lda=%ld, ldb=%ld)", d.lda, d.ldb);
+    // 16-byte loads need 16-byte aligned rows and even k / n boundaries
+    const bool a_ok = aligned16(d.A) && (d.lda % 2 == 0) && (d.k % 2 == 0) &&
                       (d.panel <= 0 || d.panel >= d.k || d.panel % 2 == 0);
-    const bool b_ok = aligned16(d.B) && (d.ldb % 2 == 0);
+    const bool b_ok = aligned16(d.B) && (d.ldb % 2 == 0) && (d.n % 2 == 0);
     if (!a_ok || !b_ok) vec2 = false;
     total = d.tile_start + ((d.m + ce.bm - 1) / ce.bm) * d.tiles_n;
   }
+  if (mp && !ce.panels)
+    return phpc_set_error(PHPC_ERR_ARG, "FP64 config %d does not support panel grouping", config_id);
   if (total <= 0) return PHPC_OK;
   GemmBatch b{};
   b.descs = descs_dev;
   b.ndesc = ndesc;
   b.total_tiles = total;
-  b.group_m = 8;
+  b.group_m = g_group_m;
+  b.xcd_remap = g_xcd_remap;
   b.d0 = descs_host[0];
   int G = grid;
   if (G <= 0) {
@@ -411,7 +768,8 @@ int phpc_f64_grouped_dev(const GemmDesc *descs_dev, const GemmDesc *descs_host, 
     if (G > total) G = total;
   }
   if (G < 1) G = 1;
-  hipLaunchKernelGGL(ce.fn[mp][vec2], dim3(G), dim3(ce.threads), 0, stream, b);
+  hipLaunchKernelGGL(ce.fn[mp][vec2], dim3(G), dim3(kernel_threads(config_id, mp, vec2)), ce.lds_bytes[vec2],
+                     stream, b);
   PHPC_HIP(hipGetLastError());
   return PHPC_OK;
 }
@@ -462,4 +820,30 @@ int phpc_f64_grouped(const GemmDesc *descs_in, int ndesc, int grid, int config_i
     dev = g_desc_dev;
   }
   return phpc_f64_grouped_dev(dev, live.data(), (int)live.size(), grid, config_id, stream);
+}
+
+// Tuning knobs for the autotuner: 0 = config for large panel-grouped problems, 1 =
+// tile-row swizzle width, 2 = XCD-contiguous tile order on/off, 3 = config for large
+// single-panel problems.
+extern "C" int phpc_set_tuning(int key, int value) {
+  switch (key) {
+    case 0:
+      if (value < 0 || value >= kNumConfigs) return phpc_set_error(PHPC_ERR_ARG, "config %d out of range", value);
+      g_big_config = value;
+      break;
+    case 1:
+      if (value < 1 || value > 64) return phpc_set_error(PHPC_ERR_ARG, "group width %d out of range", value);
+      g_group_m = value;
+      break;
+    case 2:
+      g_xcd_remap = value ? 1 : 0;
+      break;
+    case 3:
+      if (value < 0 || value >= kNumConfigs) return phpc_set_error(PHPC_ERR_ARG, "config %d out of range", value);
+      g_big_single = value;
+      break;
+    default:
+      return phpc_set_error(PHPC_ERR_ARG, "unknown tuning key %d", key);
+  }
+  return phpc_set_error(PHPC_OK, nullptr);
 }

@@ -12,7 +12,11 @@
 //             packed copy (the MPI_Type_vector send of :53-56), owners read in place
 //             with ld = N (:72-73, :82-83), then one grouped launch does every rank's
 //             update; the C blocks stay in place (the gather of :97-110 is a no-op);
-//   VENDOR    the same steps with rocBLAS dgemm as the local GEMM (phpc_gemm_cublas).
+//   VENDOR    the same steps with rocBLAS dgemm as the local GEMM (phpc_gemm_cublas);
+//   CHAINED   like FUSED, but each output is one FMA chain over all of k (the panels
+//             are walked in the same order without rounding C between them), which
+//             frees the registers that carry C and admits the 256x128 tile. The
+//             result equals the reference kernel's arithmetic for a single panel.
 // FUSED and STEPWISE run the same per-element arithmetic and give bit-identical C.
 #include "phpc_internal.h"
 #include "phpc_summa.h"
@@ -52,7 +56,7 @@ extern "C" int phpc_summa_device(int p, int q, int panel, int mode, const double
   const LaunchPolicy &pol = phpc_policy();
   const int G = wg > 0 ? wg : pol.grid_override;
 
-  if (mode == PHPC_SUMMA_FUSED) {
+  if (mode == PHPC_SUMMA_FUSED || mode == PHPC_SUMMA_CHAINED) {
     std::vector<GemmDesc> d;
     d.reserve((size_t)p * q);
     for (int i = 0; i < p; ++i)
@@ -68,7 +72,7 @@ extern "C" int phpc_summa_device(int p, int q, int panel, int mode, const double
         g.m = blk_size(N, p, i);
         g.n = blk_size(N, q, j);
         g.k = N;
-        g.panel = w;
+        g.panel = mode == PHPC_SUMMA_CHAINED ? 0 : w;
         d.push_back(g);
       }
     int rc = phpc_f64_grouped(d.data(), (int)d.size(), G, pol.config_id, s);

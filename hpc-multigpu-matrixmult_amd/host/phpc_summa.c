@@ -81,7 +81,8 @@ int phpc_grid_set_panel(phpc_grid_t grid, int panel) {
 int phpc_grid_get_panel(phpc_grid_t grid) { return grid ? grid->panel : -1; }
 
 int phpc_grid_set_mode(phpc_grid_t grid, int mode) {
-  if (!grid || (mode != PHPC_SUMMA_FUSED && mode != PHPC_SUMMA_STEPWISE)) return PHPC_ERR_ARG;
+  if (!grid || (mode != PHPC_SUMMA_FUSED && mode != PHPC_SUMMA_STEPWISE && mode != PHPC_SUMMA_CHAINED))
+    return PHPC_ERR_ARG;
   grid->mode = mode;
   return PHPC_OK;
 }

@@ -33,6 +33,7 @@ PHPC_ERR_BUDGET = 6
 SUMMA_FUSED = 0
 SUMMA_STEPWISE = 1
 SUMMA_VENDOR = 2
+SUMMA_CHAINED = 3
 
 DIST_UNIFORM = 0
 DIST_WIDE = 1
@@ -60,6 +61,7 @@ _SIGS = {
     "phpc_dgemm_vendor_device": (I, [vp, L, vp, L, vp, L, I, I, I, vp]),
     "phpc_set_launch_policy": (I, [I, I, I]),
     "phpc_num_configs": (I, []),
+    "phpc_set_tuning": (I, [I, I]),
     "phpc_config_info": (I, [I, c_int_p, c_int_p, c_int_p]),
     "phpc_fill_device": (I, [vp, L, I, I, I, ctypes.c_uint64, vp]),
     "phpc_check_sampled": (I, [vp, L, vp, L, vp, L, vp, L, I, I, I, c_int_p, c_int_p, I,

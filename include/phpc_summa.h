@@ -30,15 +30,22 @@ int phpc_grid_dims(phpc_grid_t grid, int dims[2]);
 /* k-panel width; 0 (default) selects the reference's N / lcm(p, q) (src/phpc_summa.c:36-38). */
 int phpc_grid_set_panel(phpc_grid_t grid, int panel);
 int phpc_grid_get_panel(phpc_grid_t grid);
-/* Execution mode of phpc_gemm_summa_cuda:
+/* Execution mode of phpc_gemm_summa_cuda / phpc_summa_device:
  *   PHPC_SUMMA_FUSED    one persistent launch over all ranks, walking the panels with
  *                       each rank's C tile kept in registers (default)
  *   PHPC_SUMMA_STEPWISE per panel: pack the broadcast panels (strided -> packed, as the
  *                       MPI_Type_vector sends do), then one grouped local-update launch
- * Both produce bit-identical C. */
+ *   PHPC_SUMMA_VENDOR   the STEPWISE schedule with rocBLAS dgemm as the local GEMM
+ *                       (phpc_summa_device only)
+ *   PHPC_SUMMA_CHAINED  one persistent launch, one FMA chain per output over all panels
+ *                       (C is not rounded between panels): the arithmetic of a single
+ *                       local-GEMM call over the full k; the fastest FP64 schedule
+ * FUSED and STEPWISE produce bit-identical C (the reference's per-panel grouping). */
 #define PHPC_SUMMA_FUSED 0
 #define PHPC_SUMMA_STEPWISE 1
-int phpc_grid_set_mode(phpc_grid_t grid, int mode);
+#define PHPC_SUMMA_VENDOR 2
+#define PHPC_SUMMA_CHAINED 3
+int phpc_grid_set_mode(phpc_grid_t grid, int mode); /* FUSED, STEPWISE or CHAINED */
 
 /* Least common multiple, src/phpc_summa.c:9-22. */
 int find_lcm(int a, int b);
@@ -76,10 +83,8 @@ int phpc_gemm_summa(gemm_t f, phpc_grid_t grid, const double *A, const double *B
 
 /* ---------------- device-resident extensions ---------------- */
 
-/* SUMMA on device pointers (N x N, row-major, ld = N), stream-ordered.
- * mode: PHPC_SUMMA_FUSED, PHPC_SUMMA_STEPWISE, or PHPC_SUMMA_VENDOR (rocBLAS per
- * panel and rank). panel 0 = N / lcm(p, q). wg > 0 fixes the persistent grid. */
-#define PHPC_SUMMA_VENDOR 2
+/* SUMMA on device pointers (N x N, row-major, ld = N), stream-ordered, any mode above.
+ * panel 0 = N / lcm(p, q). wg > 0 fixes the persistent grid. */
 int phpc_summa_device(int p, int q, int panel, int mode, const double *dA, const double *dB, double *dC, int N,
                       int wg, void *stream);
 

@@ -115,3 +115,37 @@ def test_summa_full_size_properties(dist):
     cw, nw = phpc.check_sampled(dA, dB, dC1, rows, cols)
     print(f"  err_cw={cw:.3e} err_nw={nw:.3e}", flush=True)
     assert cw <= 1e-14
+
+
+@pytest.mark.parametrize("p,q,N", [(4, 4, 512), (3, 3, 301), (2, 4, 258)])
+def test_summa_chained_equals_single_panel_kernel(p, q, N):
+    """CHAINED mode: one FMA chain per output over all of k, i.e. the reference kernel's
+    arithmetic for one local-GEMM call over the full k."""
+    import torch
+    A, B, C0 = O.uniform((N, N), 11), O.uniform((N, N), 12), O.uniform((N, N), 13)
+    dC = torch.from_numpy(C0.copy()).cuda()
+    phpc.summa_device(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(), dC, p, q, panel=16,
+                      mode=phpc.SUMMA_CHAINED)
+    torch.cuda.synchronize()
+    want = O.kernel_emul(A, B, C0.copy())
+    assert np.array_equal(bits(dC.cpu().numpy()), bits(want))
+
+
+def test_large_single_gemm_uses_wide_tile_and_stays_exact():
+    """N = 4096 selects the 256x128 tile: bit-exact against the kernel emulation on
+    sampled rows (the emulation of full rows is cheap)."""
+    import torch
+    N = 4096
+    dA = torch.empty((N, N), dtype=torch.float64, device="cuda")
+    dB = torch.empty_like(dA)
+    phpc.fill_device(dA, phpc.DIST_UNIFORM, 5)
+    phpc.fill_device(dB, phpc.DIST_UNIFORM, 6)
+    dC = torch.zeros_like(dA)
+    phpc.dgemm_device(dA, dB, dC)
+    torch.cuda.synchronize()
+    A = O.uniform((N, N), 5)
+    B = O.uniform((N, N), 6)
+    rows = [0, 1, 255, 256, 2049, 4095]
+    got = dC[rows].cpu().numpy()
+    want = O.kernel_emul(np.ascontiguousarray(A[rows]), B, np.zeros((len(rows), N)))
+    assert np.array_equal(bits(got), bits(want))
