@@ -89,6 +89,8 @@ def main():
     ap.add_argument("--panel", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-vendor", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a 1 GiB device-to-device copy three times (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build emulates the process grid on a single "
@@ -97,6 +99,16 @@ def main():
 
     import numpy as np
     import torch
+
+    if args.copy_probe:
+        n = (1 << 30) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n * 8, "copies": 3}))
+        return
 
     phpc = load_phpc()
     N, panel = args.n, args.panel

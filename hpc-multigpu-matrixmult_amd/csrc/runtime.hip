@@ -49,7 +49,7 @@ struct ScratchSlot {
   void *p = nullptr;
   size_t cap = 0;
 };
-ScratchSlot g_scratch[8];
+ScratchSlot g_scratch[16];
 
 void do_init() {
   int n = 0;

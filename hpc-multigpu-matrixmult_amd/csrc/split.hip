@@ -1,0 +1,913 @@
+// Split-precision FP64 GEMM (Ozaki scheme, integer slices) on gfx950 int8 matrix cores.
+//
+// C[m x n] += A[m x k] * B[k x n], FP64 in and out, computed from exact int8 slices:
+//
+//   A[i,l] = 2^e_i * sum_{t=1..s} a_t[i,l] 2^{-7t} + (truncation),  a_t in [-127, 127]
+//   B[l,j] = 2^f_j * sum_{u=1..s} b_u[l,j] 2^{-7u} + (truncation)
+//
+// with e_i (f_j) the binary exponent of row i of A (column j of B): |A[i,:]| < 2^e_i.
+// The digits come from repeated exact steps x <- 2^7 x, d = trunc(x), x <- x - d.
+// Keeping the products with t + u <= s + 1 (triangular truncation, s(s+1)/2 of them),
+//
+//   C[i,j] += 2^(e_i + f_j) * sum_{g=2..s+1} 2^{-7g} S_g[i,j],   S_g = sum_{t+u=g} a_t b_u.
+//
+// Each S_g is ONE int8 GEMM: A's slices are stored side by side along k (row i of
+// A8 = [a_1 | a_2 | ... | a_s], each slice padded to kp bytes) and B's slices
+// transposed (row j of B8 = column j of B) in reverse order ([b_s | ... | b_1]), so
+// S_g = A8[:, 0 : (g-1)kp] . B8[:, (s-g+1)kp : s kp]^T, a contiguous k range in both.
+// int8 products accumulate exactly in int32 as long as a launch sums at most 133143
+// products per output (127^2 * 133143 < 2^31); longer ranges are split into pieces.
+// The FP64 epilogue adds the levels smallest first into an FP64 accumulator and the
+// last one applies the exact scale 2^(e_i + f_j) and adds into C.
+#include "phpc_internal.h"
+#include "phpc_split.h"
+
+#include <algorithm>
+#include <cmath>
+#include <mutex>
+#include <vector>
+
+namespace {
+
+typedef int i4v __attribute__((ext_vector_type(4)));
+typedef int i16v __attribute__((ext_vector_type(16)));
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+typedef __attribute__((address_space(3))) void lds_void;
+typedef double __attribute__((address_space(1))) gdouble;
+
+constexpr uint32_t kOOB = 0x80000000u;
+constexpr int kMaxSlices = 12;
+constexpr long kMaxExactTerms = 131072;  // products per int32 accumulation (127^2 * 2^17 < 2^31)
+// residues reach -128: 128^2 * 131008 < 2^31 (131008 = 2^17 - 64, a whole number of slabs)
+constexpr long kMaxExactTermsCrt = 131008;
+
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ long uni(long v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+  return (long)(((uint64_t)hi << 32) | lo);
+}
+template <typename T>
+__device__ __forceinline__ T *uni(T *p) {
+  return reinterpret_cast<T *>(uni((long)reinterpret_cast<uintptr_t>(p)));
+}
+__device__ __forceinline__ rsrc_t make_rsrc(const void *base, long bytes) {
+  const uint32_t nb = bytes > 0x7fffffffL ? 0x7fffffffu : (uint32_t)(bytes < 0 ? 0 : bytes);
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)nb, 0x00020000);
+}
+template <int N>
+__device__ __forceinline__ void wait_vm_c() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt(0x0F70 | (N & 0xF) | (((N >> 4) & 3) << 14));
+}
+
+// ---------------------------------------------------------------------------
+// slicing
+// ---------------------------------------------------------------------------
+
+// Binary exponent E with |x| < 2^E for every x of the row/column (max = m * 2^E,
+// m in [0.5, 1)); 0 for an all-zero vector.
+__device__ __forceinline__ int exp_of_max(double mx) {
+  int e = 0;
+  if (mx > 0.0) (void)frexp(mx, &e);
+  return e;
+}
+
+// One workgroup per row of A: exponent e_i, 2^e_i / mean|a_i.|, non-finite flag.
+__global__ __launch_bounds__(256) void row_stats_kernel(const double *__restrict__ A, long lda, int k,
+                                                        int *__restrict__ eA, double *__restrict__ ratioA,
+                                                        int *__restrict__ bad) {
+  __shared__ double red_max[256];
+  __shared__ double red_sum[256];
+  const int i = blockIdx.x;
+  const double *row = A + (long)i * lda;
+  double mx = 0.0, sm = 0.0;
+  for (int l = threadIdx.x; l < k; l += 256) {
+    const double v = fabs(row[l]);
+    if (!(v <= 1.79769313486231570815e308)) *bad = 1;  // inf or nan
+    mx = fmax(mx, v);
+    sm += v;
+  }
+  red_max[threadIdx.x] = mx;
+  red_sum[threadIdx.x] = sm;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      red_max[threadIdx.x] = fmax(red_max[threadIdx.x], red_max[threadIdx.x + off]);
+      red_sum[threadIdx.x] += red_sum[threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const int e = exp_of_max(red_max[0]);
+    eA[i] = e;
+    ratioA[i] = red_sum[0] > 0.0 ? ldexp((double)k, e) / red_sum[0] : 0.0;
+  }
+}
+
+// A8 row i: slice t (1-based) at bytes [(t-1)*kp, (t-1)*kp + k), zero up to kp.
+__global__ __launch_bounds__(256) void slice_rows_kernel(const double *__restrict__ A, long lda, int k, int s,
+                                                         long kp, const int *__restrict__ eA,
+                                                         int8_t *__restrict__ A8, long ld8) {
+  const int i = blockIdx.x;
+  const double *row = A + (long)i * lda;
+  const int e = eA[i];
+  int8_t *out = A8 + (long)i * ld8;
+  for (long l = threadIdx.x; l < kp; l += 256) {
+    double x = l < k ? ldexp(row[l], -e) : 0.0;
+    for (int t = 0; t < s; ++t) {
+      x = ldexp(x, 7);
+      const double d = trunc(x);
+      x -= d;
+      out[(long)t * kp + l] = (int8_t)(int)d;
+    }
+  }
+}
+
+__global__ void zero_words_kernel(uint32_t *p, long nwords) {
+  for (long w = (long)blockIdx.x * blockDim.x + threadIdx.x; w < nwords; w += (long)gridDim.x * blockDim.x) p[w] = 0u;
+}
+
+// Column maxima / sums of B over a chunk of rows; combined with atomics.
+__global__ __launch_bounds__(256) void col_stats_kernel(const double *__restrict__ B, long ldb, int k, int n,
+                                                        int rows_per_block, unsigned long long *__restrict__ cmax,
+                                                        double *__restrict__ csum, int *__restrict__ bad) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= n) return;
+  const int r0 = blockIdx.y * rows_per_block;
+  const int r1 = min(k, r0 + rows_per_block);
+  double mx = 0.0, sm = 0.0;
+  for (int r = r0; r < r1; ++r) {
+    const double v = fabs(B[(long)r * ldb + j]);
+    mx = fmax(mx, v);
+    sm += v;
+  }
+  if (!(mx <= 1.79769313486231570815e308)) *bad = 1;
+  // non-negative doubles order like their bit patterns
+  atomicMax(&cmax[j], (unsigned long long)__double_as_longlong(mx));
+  atomicAdd(&csum[j], sm);
+}
+
+__global__ void col_exps_kernel(const unsigned long long *__restrict__ cmax, const double *__restrict__ csum, int k,
+                                int n, int *__restrict__ eB, double *__restrict__ ratioB) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= n) return;
+  const double mx = __longlong_as_double((long long)cmax[j]);
+  const int e = exp_of_max(mx);
+  eB[j] = e;
+  ratioB[j] = csum[j] > 0.0 ? ldexp((double)k, e) / csum[j] : 0.0;
+}
+
+// B slices, transposed: B8 row j, slice u at bytes [(s-u)*kp, (s-u)*kp + k), zero to kp.
+// 64 (k) x 64 (j) tiles go through LDS so both the read of B and the write of B8 are
+// coalesced.
+__global__ __launch_bounds__(256) void slice_cols_kernel(const double *__restrict__ B, long ldb, int k, int n, int s,
+                                                         long kp, const int *__restrict__ eB, int8_t *__restrict__ B8,
+                                                         long ld8) {
+  __shared__ double tile[64][65];
+  const int k0 = blockIdx.x * 64;
+  const int j0 = blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int r = ty; r < 64; r += 4) {
+    const int kk = k0 + r, jj = j0 + tx;
+    tile[r][tx] = (kk < k && jj < n) ? B[(long)kk * ldb + jj] : 0.0;
+  }
+  __syncthreads();
+  // each thread: column j0 + (row of this loop), 64 consecutive k
+  for (int c = ty; c < 64; c += 4) {
+    const int jj = j0 + c;
+    if (jj >= n) continue;
+    const int kk = k0 + tx;
+    if (kk >= kp) continue;
+    double x = ldexp(tile[tx][c], -eB[jj]);
+    int8_t *out = B8 + (long)jj * ld8;
+    for (int u = 1; u <= s; ++u) {
+      x = ldexp(x, 7);
+      const double d = trunc(x);
+      x -= d;
+      out[(long)(s - u) * kp + kk] = (int8_t)(int)d;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// int8 level GEMM with FP64 epilogue
+// ---------------------------------------------------------------------------
+// S[i,j] = sum_{l < kl} A8[i, a0 + l] * B8[j, b0 + l]   (exact int32)
+// epilogue: v = S * 2^scale_exp;  first ? acc = v : acc += v   (acc: FP64 m x n, ld n)
+//           last: C[i,j] += 2^(eA[i]+eB[j]) * acc   (acc not written)
+struct LevelArgs {
+  const int8_t *A8;
+  const int8_t *B8;
+  long lda8, ldb8;  // bytes
+  long a0, b0;      // k offsets (bytes) of this level piece
+  int m, n, kl;
+  int scale_exp;
+  double *acc;
+  long ldacc;
+  double *C;
+  long ldc;
+  const int *eA;
+  const int *eB;
+  int first, last;
+  // residue mode (CRT scheme): res[i,j] = smod(S + (first ? 0 : res[i,j]), mod_m)
+  int mod_m;  // 0: level mode (FP64 epilogue above)
+  int8_t *res;
+  long ldres;
+};
+
+// x mod m in the symmetric range (m odd: |r| <= (m-1)/2; m = 256: [-128, 127]) for
+// integer-valued doubles |x| < 2^52, by an exact floor division in double.
+__device__ __forceinline__ int smod_d(double x, int m) {
+  const double md = (double)m;
+  double q = floor(x / md);
+  double r = x - q * md;  // exact: |q * m| < 2^53
+  if (r < 0.0) r += md;
+  if (r >= md) r -= md;
+  int ri = (int)r;
+  if (ri > (m - 1) / 2) ri -= m;
+  return ri;
+}
+
+// LDS layout per stage: A then B, BM (BN) rows of BK k-bytes each, in 16-byte chunks.
+// Chunk c of row r sits at slot c ^ sw(r), sw(r) = (r * CPR / 16) mod CPR (CPR chunks per
+// row), so the 16-byte operand reads of 16 consecutive rows cover 16 distinct 16-byte
+// bank groups. Level ranges are whole multiples of 64 bytes, so every slab is full.
+template <int BM, int BN, int WM, int WN, int BK_, int STAGES>
+struct I8Cfg {
+  static constexpr int BK = BK_;        // bytes of k per slab
+  static constexpr int CPR = BK_ / 16;  // chunks per row
+  static constexpr int KSTEPS = BK_ / 32;
+  static constexpr int NW = WM * WN;
+  static constexpr int WTM = BM / WM, WTN = BN / WN;
+  static constexpr int TM = WTM / 32, TN = WTN / 32;
+  static constexpr int A_BLOCKS = BM * CPR / 64, B_BLOCKS = BN * CPR / 64;
+  static constexpr int A_BYTES = A_BLOCKS * 1024, B_BYTES = B_BLOCKS * 1024;
+  static constexpr int STAGE_BYTES = A_BYTES + B_BYTES;
+  static constexpr int BLOCKS = A_BLOCKS + B_BLOCKS;
+  static constexpr int PER = BLOCKS / NW;
+  static constexpr int ROWS_PER_BLOCK = 64 / CPR;
+  static_assert(BK_ == 64 || BK_ == 128, "slab depth");
+  static_assert(BLOCKS % NW == 0 && (BM * CPR) % 64 == 0 && (BN * CPR) % 64 == 0, "blocks must split evenly");
+  static_assert(STAGES * STAGE_BYTES <= 160 * 1024, "LDS budget");
+  static_assert(TM >= 1 && TN >= 1, "wave tile");
+};
+
+template <int CPR>
+__device__ __forceinline__ int i8_swz(int r) {
+  return (r * CPR / 16) & (CPR - 1);
+}
+
+template <int BM, int BN, int WM, int WN, int BK_, int STAGES>
+__device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
+  using K = I8Cfg<BM, BN, WM, WN, BK_, STAGES>;
+  extern __shared__ __attribute__((aligned(1024))) char smem[];
+  const int tid = threadIdx.x;
+  const int wave = uni(tid >> 6);
+  const int lane = tid & 63;
+  const int wm = wave / WN, wn = wave % WN;
+
+  const int m = uni(args.m), n = uni(args.n), kl = uni(args.kl);
+  const int tiles_n = (n + BN - 1) / BN;
+  const int tiles_m = (m + BM - 1) / BM;
+  const int total = tiles_m * tiles_n;
+  const int G = gridDim.x;
+  const int bid = blockIdx.x;
+  const int slot = uni((G % 8) == 0 ? (bid % 8) * (G / 8) + bid / 8 : bid);
+  const long lda8 = uni(args.lda8), ldb8 = uni(args.ldb8);
+  const int8_t *A8 = uni(args.A8) + uni(args.a0);
+  const int8_t *B8 = uni(args.B8) + uni(args.b0);
+  const int nslab = kl / K::BK;
+
+  // LDS operand offsets per k step: lane reads 16 bytes of row (l % 32), chunk
+  // 2*kk + l/32, at its swizzled slot
+  int a_off[K::KSTEPS][K::TM], b_off[K::KSTEPS][K::TN];
+#pragma unroll
+  for (int kk = 0; kk < K::KSTEPS; ++kk) {
+    const int c = 2 * kk + (lane >> 5);
+#pragma unroll
+    for (int i = 0; i < K::TM; ++i) {
+      const int r = wm * K::WTM + i * 32 + (lane & 31);
+      a_off[kk][i] = r * K::BK + ((c ^ i8_swz<K::CPR>(r)) << 4);
+    }
+#pragma unroll
+    for (int j = 0; j < K::TN; ++j) {
+      const int r = wn * K::WTN + j * 32 + (lane & 31);
+      b_off[kk][j] = K::A_BYTES + r * K::BK + ((c ^ i8_swz<K::CPR>(r)) << 4);
+    }
+  }
+
+  for (int round = 0;; ++round) {
+    const int t = round * G + slot;
+    if (t >= total) break;
+    // group 8 tile-rows for L2 reuse
+    const int gm = 8;
+    const int grp = t / (gm * tiles_n);
+    const int first_m = grp * gm;
+    const int gsz = min(tiles_m - first_m, gm);
+    const int in_grp = t - grp * gm * tiles_n;
+    const int row0 = uni((first_m + in_grp % gsz) * BM);
+    const int col0 = uni((in_grp / gsz) * BN);
+    const int m_rem = m - row0, n_rem = n - col0;
+    const rsrc_t rA = make_rsrc(A8 + (long)row0 * lda8, (long)min(m_rem, BM) * lda8);
+    const rsrc_t rB = make_rsrc(B8 + (long)col0 * ldb8, (long)min(n_rem, BN) * ldb8);
+
+    // this wave's blocks: wave, wave + NW, ...; lane p of a block fills slot p % CPR of
+    // row p / CPR, i.e. chunk (p % CPR) ^ sw(row); offsets for k = 0 (kb in soffset)
+    uint32_t goff[K::PER];
+#pragma unroll
+    for (int u = 0; u < K::PER; ++u) {
+      const int blk = wave + u * K::NW;
+      const bool isA = blk < K::A_BLOCKS;
+      const int r = (isA ? blk : blk - K::A_BLOCKS) * K::ROWS_PER_BLOCK + lane / K::CPR;
+      const int c = (lane % K::CPR) ^ i8_swz<K::CPR>(r);
+      const int rows = isA ? min(m_rem, BM) : min(n_rem, BN);
+      const long ld = isA ? lda8 : ldb8;
+      goff[u] = r < rows ? (uint32_t)((long)r * ld + c * 16) : kOOB;
+    }
+
+    auto issue = [&](int sidx) {
+      const int kb = sidx * K::BK;
+      char *stage = smem + (sidx % STAGES) * K::STAGE_BYTES;
+#pragma unroll
+      for (int u = 0; u < K::PER; ++u) {
+        const int blk = wave + u * K::NW;
+        // the range check applies to the lane offset (kOOB reads zero); kb rides in soffset
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(blk < K::A_BLOCKS ? rA : rB, (lds_void *)(stage + blk * 1024), 16,
+                                                  goff[u], kb, 0, 0);
+      }
+    };
+
+    i16v acc[K::TM][K::TN];
+#pragma unroll
+    for (int i = 0; i < K::TM; ++i)
+#pragma unroll
+      for (int j = 0; j < K::TN; ++j) acc[i][j] = i16v{};
+
+    __syncthreads();
+#pragma unroll
+    for (int s0 = 0; s0 < STAGES - 1; ++s0)
+      if (s0 < nslab) issue(s0);
+    for (int sidx = 0; sidx < nslab; ++sidx) {
+      const int later = min(STAGES - 2, nslab - 1 - sidx);
+      if (later <= 0)
+        wait_vm_c<0>();
+      else if (later == 1)
+        wait_vm_c<K::PER>();
+      else
+        wait_vm_c<2 * K::PER>();
+      __syncthreads();
+      if (sidx + STAGES - 1 < nslab) issue(sidx + STAGES - 1);
+      const char *stage = smem + (sidx % STAGES) * K::STAGE_BYTES;
+#pragma unroll
+      for (int kk = 0; kk < K::KSTEPS; ++kk) {
+        i4v a[K::TM], b[K::TN];
+#pragma unroll
+        for (int i = 0; i < K::TM; ++i) a[i] = *reinterpret_cast<const i4v *>(stage + a_off[kk][i]);
+#pragma unroll
+        for (int j = 0; j < K::TN; ++j) b[j] = *reinterpret_cast<const i4v *>(stage + b_off[kk][j]);
+#pragma unroll
+        for (int i = 0; i < K::TM; ++i)
+#pragma unroll
+          for (int j = 0; j < K::TN; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[i], b[j], acc[i][j], 0, 0, 0);
+      }
+    }
+
+    // epilogue: 32x32 block, lane l / reg r -> row 8(r/4) + 4(l/32) + r%4, column l%32
+    const int mod_m = uni(args.mod_m);
+    if (mod_m > 0) {
+      const int firstr = uni(args.first);
+      int8_t *resg = uni(args.res);
+      const long ldres = uni(args.ldres);
+#pragma unroll
+      for (int i = 0; i < K::TM; ++i)
+#pragma unroll
+        for (int j = 0; j < K::TN; ++j) {
+          const int col = col0 + wn * K::WTN + j * 32 + (lane & 31);
+          if (col >= n) continue;
+#pragma unroll
+          for (int r = 0; r < 16; ++r) {
+            const int row = row0 + wm * K::WTM + i * 32 + 8 * (r / 4) + 4 * (lane >> 5) + (r % 4);
+            if (row >= m) continue;
+            int8_t *pr = resg + (long)row * ldres + col;
+            double v = (double)acc[i][j][r];
+            if (!firstr) v += (double)*pr;
+            *pr = (int8_t)smod_d(v, mod_m);
+          }
+        }
+      continue;
+    }
+    const int sc = uni(args.scale_exp);
+    const int first = uni(args.first), last = uni(args.last);
+    gdouble *accg = (gdouble *)uni(args.acc);
+    gdouble *Cg = (gdouble *)uni(args.C);
+    const long ldacc = uni(args.ldacc), ldc = uni(args.ldc);
+#pragma unroll
+    for (int i = 0; i < K::TM; ++i)
+#pragma unroll
+      for (int j = 0; j < K::TN; ++j) {
+        const int col = col0 + wn * K::WTN + j * 32 + (lane & 31);
+        if (col >= n) continue;
+        const int ej = last ? args.eB[col] : 0;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int row = row0 + wm * K::WTM + i * 32 + 8 * (r / 4) + 4 * (lane >> 5) + (r % 4);
+          if (row >= m) continue;
+          double v = ldexp((double)acc[i][j][r], sc);
+          gdouble *pa = accg + (long)row * ldacc + col;
+          if (!first) v = *pa + v;
+          if (last) {
+            gdouble *pc = Cg + (long)row * ldc + col;
+            *pc = *pc + ldexp(v, args.eA[row] + ej);
+          } else {
+            *pa = v;
+          }
+        }
+      }
+  }
+}
+
+__global__ __launch_bounds__(256) void i8_level_256x256_w4(LevelArgs a) { i8_level_body<256, 256, 2, 2, 64, 3>(a); }
+__global__ __launch_bounds__(512) void i8_level_256x256_w8(LevelArgs a) { i8_level_body<256, 256, 2, 4, 64, 3>(a); }
+__global__ __launch_bounds__(256) void i8_level_128x128_w4(LevelArgs a) { i8_level_body<128, 128, 2, 2, 64, 4>(a); }
+__global__ __launch_bounds__(512) void i8_level_256x256_w8_k128(LevelArgs a) {
+  i8_level_body<256, 256, 2, 4, 128, 2>(a);
+}
+__global__ __launch_bounds__(256) void i8_level_256x256_w4_k128(LevelArgs a) {
+  i8_level_body<256, 256, 2, 2, 128, 2>(a);
+}
+__global__ __launch_bounds__(512) void i8_level_256x128_w8(LevelArgs a) { i8_level_body<256, 128, 4, 2, 64, 4>(a); }
+
+typedef void (*LevelFn)(LevelArgs);
+struct I8Config {
+  int bm, bn, threads;
+  size_t lds;
+  LevelFn fn;
+};
+#define I8CFG(BM_, BN_, WM_, WN_, BK_, ST_, FN_) \
+  I8Config { BM_, BN_, WM_ *WN_ * 64, (size_t)(ST_)*I8Cfg<BM_, BN_, WM_, WN_, BK_, ST_>::STAGE_BYTES, FN_ }
+const I8Config kI8Configs[] = {
+    I8CFG(256, 256, 2, 2, 64, 3, i8_level_256x256_w4),        // 0: 4 waves, wave tile 128x128
+    I8CFG(256, 256, 2, 4, 64, 3, i8_level_256x256_w8),        // 1: 8 waves, wave tile 128x64
+    I8CFG(128, 128, 2, 2, 64, 4, i8_level_128x128_w4),        // 2: 4 waves, wave tile 64x64 (small problems)
+    I8CFG(256, 256, 2, 4, 128, 2, i8_level_256x256_w8_k128),  // 3: 8 waves, 128-byte slabs
+    I8CFG(256, 256, 2, 2, 128, 2, i8_level_256x256_w4_k128),  // 4: 4 waves, 128-byte slabs
+    I8CFG(256, 128, 4, 2, 64, 4, i8_level_256x128_w8),        // 5: 8 waves, wave tile 64x64
+};
+constexpr int kNumI8 = sizeof(kI8Configs) / sizeof(kI8Configs[0]);
+int g_i8_big = 3;
+
+std::mutex g_mu;
+bool g_ready = false;
+int g_occ[kNumI8];
+
+void init_locked() {
+  if (g_ready) return;
+  for (int c = 0; c < kNumI8; ++c) {
+    const void *fn = reinterpret_cast<const void *>(kI8Configs[c].fn);
+    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kI8Configs[c].lds);
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, kI8Configs[c].threads, kI8Configs[c].lds) != hipSuccess ||
+        n < 1)
+      n = 1;
+    g_occ[c] = n;
+  }
+  g_ready = true;
+}
+
+int launch_level(const LevelArgs &a, int cfg, hipStream_t s) {
+  const I8Config &c = kI8Configs[cfg];
+  int occ;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    init_locked();
+    occ = g_occ[cfg];
+  }
+  const long tiles = (long)((a.m + c.bm - 1) / c.bm) * ((a.n + c.bn - 1) / c.bn);
+  long G = (long)occ * phpc_device().cus;
+  if (G > tiles) G = tiles;
+  if (G < 1) G = 1;
+  hipLaunchKernelGGL(c.fn, dim3((unsigned)G), dim3(c.threads), c.lds, s, a);
+  PHPC_HIP(hipGetLastError());
+  return PHPC_OK;
+}
+
+// Error estimate used to pick the slice count: the truncated terms of one product are
+// below (s+1) 2^{-7s} 2^{e_i+f_j}; they add up like a random walk over k, while
+// (|A||B|)_ij ~ k * mean|a_i.| * mean|b_.j| = k 2^{e_i+f_j} / (ratio_i ratio_j).
+double split_error_estimate(int s, double ratio_a, double ratio_b, int k) {
+  return (s + 1) * std::ldexp(1.0, -7 * s) * ratio_a * ratio_b / std::sqrt((double)std::max(k, 1));
+}
+
+// ---------------------------------------------------------------------------
+// CRT scheme (Ozaki scheme II): integer operands, int8 residues, Garner reconstruction
+// ---------------------------------------------------------------------------
+// A' = rint(A[i,l] 2^(p - e_i)) and B' = rint(B[l,j] 2^(p - f_j)) are integers of at most
+// p+1 bits; C' = A' B' is computed exactly from its residues modulo L pairwise coprime
+// moduli m_t <= 256 (residues fit int8), one int8 GEMM per modulus, and
+// C += 2^(e_i + f_j - 2p) C'. The moduli product M must exceed 2 max|C'|, i.e.
+// sum log2 m_t >= 2p + ceil(log2 k) + 2.
+constexpr int kMaxMods = 20;
+const int kModuli[kMaxMods] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227,
+                               223, 217, 211, 199, 197, 193, 191, 181, 179, 173};
+
+struct CrtConsts {
+  int L;
+  int m[kMaxMods];
+  int c24[kMaxMods];           // 2^24 mod m
+  int c48[kMaxMods];           // 2^48 mod m
+  int inv[kMaxMods];           // (m_1 ... m_{t-1})^-1 mod m_t
+  int pmod[kMaxMods][kMaxMods];  // pmod[s][t] = (m_1 ... m_{s-1}) mod m_t, s < t
+};
+
+// residue of an integer-valued double |x| < 2^72 modulo m, symmetric range
+__device__ __forceinline__ int8_t residue_of(double x, const CrtConsts &cc, int t, double h, double mid, double lo) {
+  (void)x;
+  const double sum = h * (double)cc.c48[t] + mid * (double)cc.c24[t] + lo;  // < 2^34, exact
+  return (int8_t)smod_d(sum, cc.m[t]);
+}
+
+__device__ __forceinline__ void split24(double ax, double &h, double &mid, double &lo) {
+  h = trunc(ldexp(ax, -48));
+  const double rest = ax - ldexp(h, 48);
+  mid = trunc(ldexp(rest, -24));
+  lo = rest - ldexp(mid, 24);
+}
+
+// A8 row i: residue plane t at bytes [t*kp, t*kp + k), zero to kp
+__global__ __launch_bounds__(256) void residue_rows_kernel(const double *__restrict__ A, long lda, int k, int p,
+                                                           long kp, const int *__restrict__ eA, CrtConsts cc,
+                                                           int8_t *__restrict__ A8, long ld8) {
+  const int i = blockIdx.x;
+  const double *row = A + (long)i * lda;
+  const int sh = p - eA[i];
+  int8_t *out = A8 + (long)i * ld8;
+  for (long l = threadIdx.x; l < kp; l += 256) {
+    const double x = l < k ? rint(ldexp(row[l], sh)) : 0.0;
+    const double ax = fabs(x);
+    double h, mid, lo;
+    split24(ax, h, mid, lo);
+    for (int t = 0; t < cc.L; ++t) {
+      int r = residue_of(ax, cc, t, h, mid, lo);
+      if (x < 0.0) r = -r;
+      if (r == 128) r = -128;  // m = 256: keep [-128, 127]
+      out[(long)t * kp + l] = (int8_t)r;
+    }
+  }
+}
+
+// B8 row j (column j of B): residue plane t at bytes [t*kp, t*kp + k)
+__global__ __launch_bounds__(256) void residue_cols_kernel(const double *__restrict__ B, long ldb, int k, int n, int p,
+                                                           long kp, const int *__restrict__ eB, CrtConsts cc,
+                                                           int8_t *__restrict__ B8, long ld8) {
+  __shared__ double tile[64][65];
+  const int k0 = blockIdx.x * 64;
+  const int j0 = blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int r = ty; r < 64; r += 4) {
+    const int kk = k0 + r, jj = j0 + tx;
+    tile[r][tx] = (kk < k && jj < n) ? B[(long)kk * ldb + jj] : 0.0;
+  }
+  __syncthreads();
+  for (int c = ty; c < 64; c += 4) {
+    const int jj = j0 + c;
+    if (jj >= n) continue;
+    const int kk = k0 + tx;
+    const double x = rint(ldexp(tile[tx][c], p - eB[jj]));
+    const double ax = fabs(x);
+    double h, mid, lo;
+    split24(ax, h, mid, lo);
+    int8_t *out = B8 + (long)jj * ld8;
+    for (int t = 0; t < cc.L; ++t) {
+      int r = residue_of(ax, cc, t, h, mid, lo);
+      if (x < 0.0) r = -r;
+      if (r == 128) r = -128;
+      out[(long)t * kp + kk] = (int8_t)r;
+    }
+  }
+}
+
+#pragma clang fp contract(off)
+// C[i,j] += 2^(e_i + f_j - 2p) * C'[i,j], C' from its residues (planes of m x n bytes):
+// Garner's mixed-radix digits v_t in symmetric ranges (exact small-integer arithmetic),
+// then C' = v_1 + m_1 (v_2 + m_2 (v_3 + ...)) evaluated in double-double.
+__global__ __launch_bounds__(256) void crt_reconstruct_kernel(const int8_t *__restrict__ res, long plane, int m, int n,
+                                                              CrtConsts cc, int p, const int *__restrict__ eA,
+                                                              const int *__restrict__ eB, double *__restrict__ C,
+                                                              long ldc) {
+  const long total = (long)m * n;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    int v[kMaxMods];
+    for (int t = 0; t < cc.L; ++t) {
+      const int r = res[(long)t * plane + e];
+      int y = 0;
+      for (int s2 = 0; s2 < t; ++s2) y += v[s2] * cc.pmod[s2][t];  // |y| < 20 * 128 * 256
+      // v_t = (r - y) * inv_t  mod m_t, symmetric
+      const int mt = cc.m[t];
+      int d = (r - y) % mt;
+      if (d < 0) d += mt;
+      long long w = (long long)d * cc.inv[t] % mt;
+      int vt = (int)w;
+      if (vt > (mt - 1) / 2) vt -= mt;
+      v[t] = vt;
+    }
+    double hi = (double)v[cc.L - 1], lo = 0.0;
+    for (int t = cc.L - 2; t >= 0; --t) {
+      // (hi, lo) = (hi, lo) * m_t + v_t
+      const double mt = (double)cc.m[t];
+      const double ph = hi * mt;
+      const double pe = fma(hi, mt, -ph);
+      const double pl = lo * mt + pe;
+      const double sh = ph + (double)v[t];
+      const double bb = sh - ph;
+      const double se = (ph - (sh - bb)) + ((double)v[t] - bb);
+      const double tl = pl + se;
+      hi = sh + tl;
+      lo = tl - (hi - sh);
+    }
+    const double cp = hi + lo;
+    const long i = e / n, j = e % n;
+    double *pc = C + i * ldc + j;
+    *pc = *pc + ldexp(cp, eA[i] + eB[j] - 2 * p);
+  }
+}
+#pragma clang fp contract(on)
+
+int modinv(int a, int m) {
+  // extended Euclid, a and m coprime
+  long t = 0, nt = 1, r = m, nr = ((a % m) + m) % m;
+  while (nr) {
+    const long q = r / nr;
+    long tmp = t - q * nt;
+    t = nt;
+    nt = tmp;
+    tmp = r - q * nr;
+    r = nr;
+    nr = tmp;
+  }
+  return (int)((t % m + m) % m);
+}
+
+CrtConsts make_crt(int L) {
+  CrtConsts c{};
+  c.L = L;
+  for (int t = 0; t < L; ++t) {
+    const int m = kModuli[t];
+    c.m[t] = m;
+    long v = 1;
+    for (int b = 0; b < 24; ++b) v = v * 2 % m;
+    c.c24[t] = (int)v;
+    c.c48[t] = (int)(v * v % m);
+    long prod = 1;
+    for (int s2 = 0; s2 < t; ++s2) {
+      c.pmod[s2][t] = (int)prod;  // (m_1..m_{s-1}) mod m_t with s = s2+1
+      prod = prod * kModuli[s2] % m;
+    }
+    c.inv[t] = t == 0 ? 1 : modinv((int)prod, m);
+  }
+  return c;
+}
+
+double crt_bits(int L) {
+  double b = 0;
+  for (int t = 0; t < L; ++t) b += std::log2((double)kModuli[t]);
+  return b;
+}
+
+// a-priori estimate for integer operands of p bits: each entry moves by at most
+// 2^(e - p - 1); the errors add like a random walk over k. The factor 2 covers the tail
+// of the maximum over all m*n outputs (the random-walk figure matches the measured
+// maximum of a few thousand sampled outputs).
+double crt_error_estimate(int p, double ratio_a, double ratio_b, int k) {
+  return 2.0 * std::ldexp(1.0, -p) * (ratio_a + ratio_b) / std::sqrt((double)std::max(k, 1));
+}
+
+int method_crt = 1;
+
+}  // namespace
+
+extern "C" int phpc_split_set_config(int cfg) {
+  if (cfg < 0 || cfg >= kNumI8) return phpc_set_error(PHPC_ERR_ARG, "split config %d out of range", cfg);
+  g_i8_big = cfg;
+  return phpc_set_error(PHPC_OK, nullptr);
+}
+
+extern "C" int phpc_split_num_configs(void) { return kNumI8; }
+
+extern "C" int phpc_split_set_method(int method) {
+  if (method != PHPC_SPLIT_SLICES && method != PHPC_SPLIT_CRT)
+    return phpc_set_error(PHPC_ERR_ARG, "unknown split method %d", method);
+  method_crt = method == PHPC_SPLIT_CRT;
+  return phpc_set_error(PHPC_OK, nullptr);
+}
+
+extern "C" double phpc_split_error_estimate(int slices, double ratio_a, double ratio_b, int k) {
+  return split_error_estimate(slices, ratio_a, ratio_b, k);
+}
+
+extern "C" int phpc_dgemm_split_device(const double *dA, long lda, const double *dB, long ldb, double *dC, long ldc,
+                                       int m, int k, int n, int slices, double target_err, void *stream_,
+                                       phpc_split_info *info) {
+  if (info) *info = phpc_split_info{};
+  if (phpc_init()) return phpc_last_error();
+  if (m < 0 || k < 0 || n < 0) return phpc_set_error(PHPC_ERR_ARG, "negative size");
+  if ((m > 0 && k > 0 && (!dA || lda < k)) || (k > 0 && n > 0 && (!dB || ldb < n)) || (m > 0 && n > 0 && (!dC || ldc < n)))
+    return phpc_set_error(PHPC_ERR_ARG, "bad pointers or leading dimensions");
+  if (!method_crt && (slices < 0 || slices > kMaxSlices))
+    return phpc_set_error(PHPC_ERR_ARG, "slices must be 0 (auto) or 1..%d", kMaxSlices);
+  if (method_crt && (slices < 0 || slices == 1 || slices > kMaxMods))
+    return phpc_set_error(PHPC_ERR_ARG, "moduli must be 0 (auto) or 2..%d", kMaxMods);
+  if (m == 0 || n == 0 || k == 0) return phpc_set_error(PHPC_OK, nullptr);
+  hipStream_t s = (hipStream_t)stream_;
+  const long kp = ((long)k + 63) / 64 * 64;
+
+  // scratch: exponents, ratios, column stats (slices and the FP64 accumulator below)
+  const size_t small = (size_t)(m + n) * (sizeof(int) + sizeof(double)) + (size_t)n * 16 + 64;
+  char *meta = (char *)phpc_scratch(small, 11);
+  if (!meta) return phpc_set_error(PHPC_ERR_MEM, "split scratch allocation failed");
+  int *eA = (int *)meta;
+  int *eB = eA + m;
+  double *ratioA = (double *)(((uintptr_t)(eB + n) + 15) & ~(uintptr_t)15);
+  double *ratioB = ratioA + m;
+  unsigned long long *cmax = (unsigned long long *)(ratioB + n);
+  double *csum = (double *)(cmax + n);
+  int *bad = (int *)(csum + n);
+
+  {
+    const long words = ((long)n * 16 + 16) / 4;  // cmax, csum and the flag
+    hipLaunchKernelGGL(zero_words_kernel, dim3((unsigned)std::min<long>((words + 255) / 256, 4096)), dim3(256), 0, s,
+                       (uint32_t *)cmax, words);
+    PHPC_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(row_stats_kernel, dim3(m), dim3(256), 0, s, dA, lda, k, eA, ratioA, bad);
+  PHPC_HIP(hipGetLastError());
+  const int rows_per_block = 256;
+  hipLaunchKernelGGL(col_stats_kernel, dim3((n + 255) / 256, (k + rows_per_block - 1) / rows_per_block), dim3(256), 0,
+                     s, dB, ldb, k, n, rows_per_block, cmax, csum, bad);
+  PHPC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(col_exps_kernel, dim3((n + 255) / 256), dim3(256), 0, s, cmax, csum, k, n, eB, ratioB);
+  PHPC_HIP(hipGetLastError());
+
+  if (method_crt) {
+    // CRT scheme: choose the integer width p, then the modulus count
+    std::vector<double> ra(m), rb(n);
+    int hbad = 0;
+    PHPC_HIP(hipMemcpyAsync(ra.data(), ratioA, sizeof(double) * m, hipMemcpyDeviceToHost, s));
+    PHPC_HIP(hipMemcpyAsync(rb.data(), ratioB, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    PHPC_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+    PHPC_HIP(hipStreamSynchronize(s));
+    if (hbad) return phpc_set_error(PHPC_ERR_ARG, "split-precision GEMM needs finite inputs");
+    const double mra = *std::max_element(ra.begin(), ra.end());
+    const double mrb = *std::max_element(rb.begin(), rb.end());
+    const int logk = (int)std::ceil(std::log2((double)std::max(k, 2)));
+    int pbits = 0, L = 0;
+    if (slices > 0) {
+      // the caller fixes the modulus count; use the widest integers it can hold
+      L = std::min(slices, kMaxMods);
+      pbits = (int)std::floor((crt_bits(L) - logk - 2) / 2);
+    } else {
+      const double target = target_err > 0 ? target_err : 1e-14;
+      for (pbits = 8; pbits < 70; ++pbits)
+        if (crt_error_estimate(pbits, mra, mrb, k) <= target) break;
+      for (L = 2; L < kMaxMods; ++L)
+        if (crt_bits(L) >= 2.0 * pbits + logk + 2) break;
+      if (crt_bits(L) < 2.0 * pbits + logk + 2) pbits = (int)std::floor((crt_bits(L) - logk - 2) / 2);
+    }
+    if (pbits < 1) return phpc_set_error(PHPC_ERR_ARG, "k = %d too large for the CRT scheme", k);
+    const CrtConsts cc = make_crt(L);
+    const long ld8 = kp * L;
+    int8_t *A8 = (int8_t *)phpc_scratch((size_t)m * ld8, 8);
+    int8_t *B8 = (int8_t *)phpc_scratch((size_t)n * ld8, 9);
+    int8_t *R8 = (int8_t *)phpc_scratch((size_t)L * m * n, 12);
+    if (!A8 || !B8 || !R8) return phpc_set_error(PHPC_ERR_MEM, "CRT scratch allocation failed");
+    hipLaunchKernelGGL(residue_rows_kernel, dim3(m), dim3(256), 0, s, dA, lda, k, pbits, kp, eA, cc, A8, ld8);
+    PHPC_HIP(hipGetLastError());
+    hipLaunchKernelGGL(residue_cols_kernel, dim3((unsigned)((kp + 63) / 64), (unsigned)((n + 63) / 64)), dim3(256),
+                       0, s, dB, ldb, k, n, pbits, kp, eB, cc, B8, ld8);
+    PHPC_HIP(hipGetLastError());
+    long tiles256 = (long)((m + 255) / 256) * ((n + 255) / 256);
+    const int cfg = tiles256 >= (long)phpc_device().cus / 2 ? g_i8_big : 2;
+    const long piece = std::min<long>(kp, kMaxExactTermsCrt);
+    int launches = 0;
+    for (int t = 0; t < L; ++t) {
+      for (long off = 0; off < kp; off += piece) {
+        LevelArgs a{};
+        a.A8 = A8;
+        a.B8 = B8;
+        a.lda8 = ld8;
+        a.ldb8 = ld8;
+        a.a0 = (long)t * kp + off;
+        a.b0 = (long)t * kp + off;
+        a.m = m;
+        a.n = n;
+        a.kl = (int)std::min(piece, kp - off);
+        a.first = off == 0 ? 1 : 0;
+        a.mod_m = cc.m[t];
+        a.res = R8 + (size_t)t * m * n;
+        a.ldres = n;
+        int rc = launch_level(a, cfg, s);
+        if (rc) return rc;
+        ++launches;
+      }
+    }
+    const long total = (long)m * n;
+    long blocks = std::min<long>((total + 255) / 256, 65536);
+    hipLaunchKernelGGL(crt_reconstruct_kernel, dim3((unsigned)blocks), dim3(256), 0, s, R8, (long)m * n, m, n, cc,
+                       pbits, eA, eB, dC, ldc);
+    PHPC_HIP(hipGetLastError());
+    if (info) {
+      info->slices = L;
+      info->products = L;
+      info->launches = launches + 7;
+      info->error_estimate = crt_error_estimate(pbits, mra, mrb, k);
+      info->method = 1;
+      info->int_bits = pbits;
+    }
+    return phpc_set_error(PHPC_OK, nullptr);
+  }
+
+  int s_use = slices;
+  double est = 0.0;
+  {
+    // exponents and ratios decide the slice count (and reveal non-finite input)
+    std::vector<double> ra(m), rb(n);
+    int hbad = 0;
+    PHPC_HIP(hipMemcpyAsync(ra.data(), ratioA, sizeof(double) * m, hipMemcpyDeviceToHost, s));
+    PHPC_HIP(hipMemcpyAsync(rb.data(), ratioB, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    PHPC_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+    PHPC_HIP(hipStreamSynchronize(s));
+    if (hbad) return phpc_set_error(PHPC_ERR_ARG, "split-precision GEMM needs finite inputs");
+    const double mra = *std::max_element(ra.begin(), ra.end());
+    const double mrb = *std::max_element(rb.begin(), rb.end());
+    if (s_use == 0) {
+      const double target = target_err > 0 ? target_err : 1e-14;
+      s_use = kMaxSlices;
+      for (int c = 2; c <= kMaxSlices; ++c)
+        if (split_error_estimate(c, mra, mrb, k) <= target) {
+          s_use = c;
+          break;
+        }
+    }
+    est = split_error_estimate(s_use, mra, mrb, k);
+  }
+  const long ld8a = kp * s_use, ld8b = kp * s_use;
+  int8_t *A8 = (int8_t *)phpc_scratch((size_t)m * ld8a, 8);
+  int8_t *B8 = (int8_t *)phpc_scratch((size_t)n * ld8b, 9);
+  double *acc = (double *)phpc_scratch((size_t)m * n * 8, 10);
+  if (!A8 || !B8 || !acc) return phpc_set_error(PHPC_ERR_MEM, "split scratch allocation failed");
+  hipLaunchKernelGGL(slice_rows_kernel, dim3(m), dim3(256), 0, s, dA, lda, k, s_use, kp, eA, A8, ld8a);
+  PHPC_HIP(hipGetLastError());
+  // B slices in reverse order: slice u at block (s_use - u)
+  {
+    dim3 grid((unsigned)((kp + 63) / 64), (unsigned)((n + 63) / 64));
+    hipLaunchKernelGGL(slice_cols_kernel, grid, dim3(256), 0, s, dB, ldb, k, n, s_use, kp, eB, B8, ld8b);
+    PHPC_HIP(hipGetLastError());
+  }
+
+  // level GEMMs, smallest level first
+  long tiles256 = (long)((m + 255) / 256) * ((n + 255) / 256);
+  const int cfg = tiles256 >= (long)phpc_device().cus / 2 ? g_i8_big : 2;
+  const long piece = kMaxExactTerms / kp * kp;  // whole slices per piece keep offsets aligned
+  const long max_piece = piece > 0 ? piece : (kMaxExactTerms / 64) * 64;
+  bool first = true;
+  int launches = 0;
+  for (int g = s_use + 1; g >= 2; --g) {
+    const long klen = (long)(g - 1) * kp;
+    for (long off = 0; off < klen; off += max_piece) {
+      const long len = std::min(max_piece, klen - off);
+      LevelArgs a{};
+      a.A8 = A8;
+      a.B8 = B8;
+      a.lda8 = ld8a;
+      a.ldb8 = ld8b;
+      a.a0 = off;
+      a.b0 = (long)(s_use - g + 1) * kp + off;
+      a.m = m;
+      a.n = n;
+      a.kl = (int)len;
+      a.scale_exp = -7 * g;
+      a.acc = acc;
+      a.ldacc = n;
+      a.C = dC;
+      a.ldc = ldc;
+      a.eA = eA;
+      a.eB = eB;
+      a.first = first ? 1 : 0;
+      a.last = (g == 2 && off + len >= klen) ? 1 : 0;
+      int rc = launch_level(a, cfg, s);
+      if (rc) return rc;
+      first = false;
+      ++launches;
+    }
+  }
+  if (info) {
+    info->slices = s_use;
+    info->products = s_use * (s_use + 1) / 2;
+    info->launches = launches + 6;
+    info->error_estimate = est;
+    info->method = 0;
+    info->int_bits = 7 * s_use;
+  }
+  return phpc_set_error(PHPC_OK, nullptr);
+}

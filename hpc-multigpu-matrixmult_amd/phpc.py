@@ -48,6 +48,11 @@ vp = ctypes.c_void_p
 L = ctypes.c_long
 I = ctypes.c_int
 
+class SplitInfo(ctypes.Structure):
+    _fields_ = [("slices", ctypes.c_int), ("products", ctypes.c_int), ("launches", ctypes.c_int),
+                ("error_estimate", ctypes.c_double), ("method", ctypes.c_int), ("int_bits", ctypes.c_int)]
+
+
 GEMM_T = ctypes.CFUNCTYPE(None, c_double_p, I, c_double_p, I, c_double_p, I, I, I, I, I, I, I, I, c_float_p)
 
 _SIGS = {
@@ -80,6 +85,13 @@ _SIGS = {
     "phpc_gemm_summa": (I, [GEMM_T, vp, vp, vp, vp, I, I, I, I, I, c_float_p]),
     "phpc_summa_device": (I, [I, I, I, I, vp, vp, vp, I, I, vp]),
     "phpc_summa_host": (I, [I, I, I, I, vp, vp, vp, I, I, c_float_p]),
+    # include/phpc_split.h
+    "phpc_dgemm_split_device": (I, [vp, L, vp, L, vp, L, I, I, I, I, ctypes.c_double, vp,
+                                    ctypes.POINTER(SplitInfo)]),
+    "phpc_split_error_estimate": (ctypes.c_double, [I, ctypes.c_double, ctypes.c_double, I]),
+    "phpc_split_set_config": (I, [I]),
+    "phpc_split_num_configs": (I, []),
+    "phpc_split_set_method": (I, [I]),
     # include/utils.h
     "get_cur_time": (ctypes.c_double, []),
 }
@@ -265,3 +277,19 @@ def set_launch_policy(honor_grid=0, config_id=-1, grid=0):
 
 def num_configs():
     return lib().phpc_num_configs()
+
+
+SPLIT_SLICES = 0
+SPLIT_CRT = 1
+
+
+def dgemm_split_device(A, B, C, slices=0, target=1e-14, stream=None, method=None):
+    """Split-precision C += A*B on torch float64 CUDA tensors; returns SplitInfo."""
+    if method is not None:
+        check(lib().phpc_split_set_method(method))
+    info = SplitInfo()
+    m, k = A.shape
+    n = B.shape[1]
+    check(lib().phpc_dgemm_split_device(dptr(A), A.stride(0), dptr(B), B.stride(0), dptr(C), C.stride(0),
+                                        m, k, n, slices, target, stream_ptr(stream), ctypes.byref(info)))
+    return info

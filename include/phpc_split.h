@@ -1,0 +1,68 @@
+/*
+ * libphpc — split-precision GEMM: FP64-accurate products from int8 matrix-core slices.
+ *
+ * No counterpart in the reference (Redy1908/HPC-MultiGPU-MatrixMult computes only in
+ * plain FP64, src/phpc_gemm.cu:6-57). Same contract as phpc_dgemm_device: row-major
+ * FP64 operands on the device, C += A*B, stream-ordered.
+ *
+ * Method (Ozaki scheme with integer slices): rows of A and columns of B are scaled by
+ * powers of two and cut exactly into `slices` signed 7-bit digits; the s(s+1)/2 slice
+ * products with t + u <= s + 1 are int8 GEMMs accumulated exactly in int32 on the
+ * MFMA units, and the levels are recombined in FP64 with the exact power-of-two scales.
+ * The result differs from the exact product only by the dropped slice terms (about
+ * 2^(-7 s) relative to the row/column scales) and a few FP64 roundings.
+ */
+#ifndef PHPC_SPLIT_H
+#define PHPC_SPLIT_H
+
+#include "phpc_gemm.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct phpc_split_info {
+  int slices;            /* slices (PHPC_SPLIT_SLICES) or moduli (PHPC_SPLIT_CRT) used */
+  int products;          /* int8 GEMMs computed: s(s+1)/2 slice products, or one per modulus */
+  int launches;          /* kernel launches issued */
+  double error_estimate; /* a-priori componentwise error estimate used to pick the size */
+  int method;            /* PHPC_SPLIT_SLICES or PHPC_SPLIT_CRT */
+  int int_bits;          /* bits kept per operand entry (7 * slices, or p of the CRT scheme) */
+} phpc_split_info;
+
+/* Methods:
+ *   PHPC_SPLIT_SLICES  Ozaki scheme I: `slices` signed 7-bit digits per entry, the
+ *                      s(s+1)/2 slice products with t + u <= s + 1
+ *   PHPC_SPLIT_CRT     Ozaki scheme II (default): entries scaled to p-bit integers, the
+ *                      exact integer product recovered from its residues modulo
+ *                      `slices` pairwise coprime moduli <= 256 (one int8 GEMM each) by
+ *                      Garner's mixed-radix algorithm; p and the modulus count follow
+ *                      from target_err (or p from a fixed modulus count) */
+#define PHPC_SPLIT_SLICES 0
+#define PHPC_SPLIT_CRT 1
+int phpc_split_set_method(int method);
+
+/*
+ * C[m x n] (ldc) += A[m x k] (lda) * B[k x n] (ldb), FP64, through int8 slices.
+ *   slices     : slices 1..12 (PHPC_SPLIT_SLICES) or moduli 2..20 (PHPC_SPLIT_CRT), or 0
+ *                to choose the smallest size whose error estimate is at most target_err
+ *   target_err : componentwise target, max |C - C*| / (|A||B|); <= 0 means 1e-14
+ *   info       : optional, filled on success
+ * Inputs must be finite (PHPC_ERR_ARG otherwise). The call synchronises the stream once
+ * (after the exponent pass) to size the slices.
+ */
+int phpc_dgemm_split_device(const double *dA, long lda, const double *dB, long ldb, double *dC, long ldc, int m,
+                            int k, int n, int slices, double target_err, void *stream, phpc_split_info *info);
+
+/* The a-priori estimate: (s+1) 2^(-7s) * ratio_a * ratio_b / sqrt(k), where ratio_a is the
+ * largest row scale 2^e_i over the row's mean |a| (ratio_b likewise for columns). */
+double phpc_split_error_estimate(int slices, double ratio_a, double ratio_b, int k);
+
+/* int8 level-GEMM tile configuration for large problems (autotuner knob). */
+int phpc_split_set_config(int cfg);
+int phpc_split_num_configs(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -62,6 +62,12 @@ def lib():
         h.orc_err_full.argtypes = [_dp, I, _dp, I, _dp, I, _dp, I, _dp, I, I, I, I, _dp, _dp]
         h.orc_ramp_closed.restype = ctypes.c_double
         h.orc_ramp_closed.argtypes = [I, I, I]
+        h.orc_split_gemm.argtypes = [_dp, I, _dp, I, _dp, I, I, I, I, I]
+        h.orc_split_digits.argtypes = [ctypes.c_double, I, I, _ip]
+        h.orc_exp_of_max.restype = I
+        h.orc_exp_of_max.argtypes = [ctypes.c_double]
+        h.orc_crt_gemm.argtypes = [_dp, I, _dp, I, _dp, I, I, I, I, I, I]
+        h.orc_crt_digits.argtypes = [_ip, I, _ip]
         h.orc_freivalds.restype = ctypes.c_double
         h.orc_freivalds.argtypes = [_dp, _dp, _dp, I, ctypes.c_uint64]
         _lib = h
@@ -202,3 +208,40 @@ def ref_iterative(A, B, C):
     h.phpc_gemm_iterative.argtypes = [_dp, _dp, _dp, I]
     h.phpc_gemm_iterative(_p(A), _p(B), _p(C), A.shape[0])
     return C
+
+
+def split_gemm(A, B, C, s):
+    """Restatement of the library's split-precision GEMM (C updated in place)."""
+    m, k = A.shape
+    n = B.shape[1]
+    lib().orc_split_gemm(_p(A), k, _p(B), n, _p(C), n, m, k, n, s)
+    return C
+
+
+def split_digits(x, e, s):
+    d = (ctypes.c_int * s)()
+    lib().orc_split_digits(float(x), e, s, d)
+    return list(d)
+
+
+def exp_of_max(mx):
+    return lib().orc_exp_of_max(float(mx))
+
+
+MODULI = [256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193, 191, 181, 179, 173]
+
+
+def crt_gemm(A, B, C, p, L):
+    """Restatement of the library's CRT split GEMM (C updated in place)."""
+    m, k = A.shape
+    n = B.shape[1]
+    lib().orc_crt_gemm(_p(A), k, _p(B), n, _p(C), n, m, k, n, p, L)
+    return C
+
+
+def crt_digits(residues):
+    L = len(residues)
+    r = (ctypes.c_int * L)(*residues)
+    v = (ctypes.c_int * L)()
+    lib().orc_crt_digits(r, L, v)
+    return list(v)

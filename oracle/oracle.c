@@ -255,3 +255,194 @@ double orc_freivalds(const double *A, const double *B, const double *C, int n, u
   free(absbx);
   return worst;
 }
+
+/* ---------------- split-precision restatement ---------------- */
+
+int orc_exp_of_max(double mx) {
+  int e = 0;
+  if (mx > 0.0) (void)frexp(mx, &e);
+  return e;
+}
+
+void orc_split_digits(double x, int e, int s, int *d) {
+  double v = ldexp(x, -e);
+  for (int t = 0; t < s; ++t) {
+    v = ldexp(v, 7);
+    double q = trunc(v);
+    v -= q;
+    d[t] = (int)q;
+  }
+}
+
+void orc_split_gemm(const double *A, int lda, const double *B, int ldb, double *C, int ldc, int m, int k, int n,
+                    int s) {
+  if (m <= 0 || n <= 0 || k <= 0 || s <= 0) return;
+  const long kp = ((long)k + 63) / 64 * 64;
+  int *eA = (int *)malloc(sizeof(int) * (size_t)m);
+  int *eB = (int *)malloc(sizeof(int) * (size_t)n);
+  signed char *a8 = (signed char *)calloc((size_t)m * (size_t)s * (size_t)k, 1); /* [i][t][l] */
+  signed char *b8 = (signed char *)calloc((size_t)n * (size_t)s * (size_t)k, 1); /* [j][u][l] */
+  for (int i = 0; i < m; ++i) {
+    double mx = 0;
+    for (int l = 0; l < k; ++l) mx = fmax(mx, fabs(A[(size_t)i * lda + l]));
+    eA[i] = orc_exp_of_max(mx);
+  }
+  for (int j = 0; j < n; ++j) {
+    double mx = 0;
+    for (int l = 0; l < k; ++l) mx = fmax(mx, fabs(B[(size_t)l * ldb + j]));
+    eB[j] = orc_exp_of_max(mx);
+  }
+  int d[64];
+  for (int i = 0; i < m; ++i)
+    for (int l = 0; l < k; ++l) {
+      orc_split_digits(A[(size_t)i * lda + l], eA[i], s, d);
+      for (int t = 0; t < s; ++t) a8[((size_t)i * s + t) * k + l] = (signed char)d[t];
+    }
+  for (int j = 0; j < n; ++j)
+    for (int l = 0; l < k; ++l) {
+      orc_split_digits(B[(size_t)l * ldb + j], eB[j], s, d);
+      for (int u = 0; u < s; ++u) b8[((size_t)j * s + u) * k + l] = (signed char)d[u];
+    }
+  /* device piece length: whole padded slices per piece, at most 131072 products */
+  long piece = 131072 / kp * kp;
+  if (piece <= 0) piece = (131072 / 64) * 64;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < n; ++j) {
+      double acc = 0.0;
+      int first = 1;
+      for (int g = s + 1; g >= 2; --g) {
+        /* level g: A slices t = 1..g-1 against B slices u = g-t, concatenated along k
+         * with stride kp (zero padding between slices), cut into pieces of `piece` */
+        const long klen = (long)(g - 1) * kp;
+        for (long off = 0; off < klen; off += piece) {
+          const long end = off + piece < klen ? off + piece : klen;
+          long long sum = 0;
+          for (long q = off; q < end; ++q) {
+            const int t = (int)(q / kp) + 1;
+            const long l = q % kp;
+            if (l >= k) continue;
+            const int u = g - t;
+            sum += (long long)a8[((size_t)i * s + (t - 1)) * k + l] * (long long)b8[((size_t)j * s + (u - 1)) * k + l];
+          }
+          double v = ldexp((double)sum, -7 * g);
+          acc = first ? v : acc + v;
+          first = 0;
+        }
+      }
+      C[(size_t)i * ldc + j] = C[(size_t)i * ldc + j] + ldexp(acc, eA[i] + eB[j]);
+    }
+  free(eA);
+  free(eB);
+  free(a8);
+  free(b8);
+}
+
+/* ---------------- CRT restatement ---------------- */
+
+static const int orc_moduli[20] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227,
+                                   223, 217, 211, 199, 197, 193, 191, 181, 179, 173};
+
+static int orc_smod(long long x, int m) {
+  long long r = x % m;
+  if (r < 0) r += m;
+  if (r > (m - 1) / 2) r -= m;
+  return (int)r;
+}
+
+static int orc_modinv(int a, int m) {
+  for (int x = 1; x < m; ++x)
+    if ((long long)a * x % m == 1) return x;
+  return 0;
+}
+
+void orc_crt_digits(const int *res, int L, int *v) {
+  for (int t = 0; t < L; ++t) {
+    const int mt = orc_moduli[t];
+    long long prod = 1, y = 0;
+    for (int s2 = 0; s2 < t; ++s2) {
+      y += (long long)v[s2] * prod; /* prod = (m_1..m_{s2}) mod m_t */
+      prod = prod * orc_moduli[s2] % mt;
+    }
+    const int inv = t == 0 ? 1 : orc_modinv((int)prod, mt);
+    long long d = (res[t] - y) % mt;
+    if (d < 0) d += mt;
+    v[t] = orc_smod(d * inv % mt, mt);
+  }
+}
+
+static int orc_res_of_double(double x, int m) {
+  /* x is an integer-valued double, |x| < 2^127 */
+  __int128 xi = (__int128)x;
+  __int128 r = xi % m;
+  if (r < 0) r += m;
+  int ri = (int)r;
+  if (ri > (m - 1) / 2) ri -= m;
+  return ri;
+}
+
+void orc_crt_gemm(const double *A, int lda, const double *B, int ldb, double *C, int ldc, int m, int k, int n,
+                  int p, int L) {
+  if (m <= 0 || n <= 0 || k <= 0 || L <= 0) return;
+  int *eA = (int *)malloc(sizeof(int) * (size_t)m);
+  int *eB = (int *)malloc(sizeof(int) * (size_t)n);
+  signed char *ra = (signed char *)malloc((size_t)m * L * k);
+  signed char *rb = (signed char *)malloc((size_t)n * L * k);
+  for (int i = 0; i < m; ++i) {
+    double mx = 0;
+    for (int l = 0; l < k; ++l) mx = fmax(mx, fabs(A[(size_t)i * lda + l]));
+    eA[i] = orc_exp_of_max(mx);
+    for (int l = 0; l < k; ++l) {
+      const double x = rint(ldexp(A[(size_t)i * lda + l], p - eA[i]));
+      for (int t = 0; t < L; ++t) {
+        int r = orc_res_of_double(x, orc_moduli[t]);
+        if (orc_moduli[t] == 256 && r == 128) r = -128;
+        ra[((size_t)i * L + t) * k + l] = (signed char)r;
+      }
+    }
+  }
+  for (int j = 0; j < n; ++j) {
+    double mx = 0;
+    for (int l = 0; l < k; ++l) mx = fmax(mx, fabs(B[(size_t)l * ldb + j]));
+    eB[j] = orc_exp_of_max(mx);
+    for (int l = 0; l < k; ++l) {
+      const double x = rint(ldexp(B[(size_t)l * ldb + j], p - eB[j]));
+      for (int t = 0; t < L; ++t) {
+        int r = orc_res_of_double(x, orc_moduli[t]);
+        if (orc_moduli[t] == 256 && r == 128) r = -128;
+        rb[((size_t)j * L + t) * k + l] = (signed char)r;
+      }
+    }
+  }
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < n; ++j) {
+      int res[20], v[20];
+      for (int t = 0; t < L; ++t) {
+        long long sum = 0;
+        for (int l = 0; l < k; ++l)
+          sum += (long long)ra[((size_t)i * L + t) * k + l] * (long long)rb[((size_t)j * L + t) * k + l];
+        res[t] = orc_smod(sum, orc_moduli[t]);
+      }
+      orc_crt_digits(res, L, v);
+      double hi = (double)v[L - 1], lo = 0.0;
+      for (int t = L - 2; t >= 0; --t) {
+        const double mt = (double)orc_moduli[t];
+        const double ph = hi * mt;
+        const double pe = fma(hi, mt, -ph);
+        const double pl = lo * mt + pe;
+        const double sh = ph + (double)v[t];
+        const double bb = sh - ph;
+        const double se = (ph - (sh - bb)) + ((double)v[t] - bb);
+        const double tl = pl + se;
+        hi = sh + tl;
+        lo = tl - (hi - sh);
+      }
+      const double cp = hi + lo;
+      C[(size_t)i * ldc + j] = C[(size_t)i * ldc + j] + ldexp(cp, eA[i] + eB[j] - 2 * p);
+    }
+  free(eA);
+  free(eB);
+  free(ra);
+  free(rb);
+}

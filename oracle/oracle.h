@@ -108,6 +108,32 @@ double orc_ramp_closed(int N, int r, int c);
  * x[j] = uniform(seed), all products in double-double. O(N^2). */
 double orc_freivalds(const double *A, const double *B, const double *C, int n, uint64_t seed);
 
+/* ---------------- split-precision (Ozaki scheme, int8 slices) ----------------
+ * Restatement of the split GEMM the library runs (no reference counterpart; the method
+ * is Ozaki, Ogita, Oishi, Rump 2012 with integer slices as in Ootomo, Ozaki, Yokota
+ * 2024): row/column power-of-two exponents, exact 7-bit digits by repeated
+ * x <- 2^7 x, d = trunc(x), x <- x - d; level sums S_g = sum_{t+u=g} a_t b_u in exact
+ * integers, split in k pieces of at most 131072 products exactly as the device splits
+ * them; FP64 recombination from level s+1 down to 2; C += 2^(e_i + f_j) * acc.
+ * Pinned by: exactness on inputs that fit the slices (small integers, powers of two),
+ * the double-double product within the truncation bound, and error decreasing with s. */
+void orc_split_gemm(const double *A, int lda, const double *B, int ldb, double *C, int ldc, int m, int k, int n,
+                    int s);
+/* Digits of one value: d[0..s-1] and exponent e such that x = 2^e sum d_t 2^{-7t} + r. */
+void orc_split_digits(double x, int e, int s, int *d);
+int orc_exp_of_max(double mx);
+
+/* ---------------- split-precision, CRT form (Ozaki scheme II) ----------------
+ * Restatement of the library's CRT split GEMM: A' = rint(A[i,l] 2^(p - e_i)),
+ * B' = rint(B[l,j] 2^(p - f_j)); the first L moduli of
+ * {256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193, 191,
+ *  181, 179, 173}; symmetric residues; exact residue products; Garner mixed-radix digits;
+ * double-double Horner; C += 2^(e_i + f_j - 2p) C'. orc_crt_exact returns the exact
+ * integer C' = A' B' (as two 64-bit halves) so tests can pin the reconstruction. */
+void orc_crt_gemm(const double *A, int lda, const double *B, int ldb, double *C, int ldc, int m, int k, int n,
+                  int p, int L);
+void orc_crt_digits(const int *residues, int L, int *digits);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,132 @@
+"""GPU parity of the split-precision (int8-slice) FP64 GEMM: bit-exact against the
+oracle's restatement of the algorithm at small sizes, FP64-level accuracy against the
+double-double product, and full-size sampled accuracy."""
+import numpy as np
+import pytest
+
+from conftest import phpc
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(x):
+    return np.ascontiguousarray(x).view(np.uint64)
+
+
+def dev(x):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(x)).cuda()
+
+
+@pytest.mark.parametrize("m,k,n,s,gen", [(1, 1, 1, 1, "u"), (5, 7, 3, 3, "u"), (64, 64, 64, 7, "u"),
+                                         (200, 300, 130, 7, "u"), (257, 129, 255, 4, "w"),
+                                         (96, 1000, 80, 8, "w"), (33, 65, 300, 2, "u")])
+def test_split_matches_restatement_bitwise(m, k, n, s, gen):
+    import torch
+    print(f"split m={m} k={k} n={n} s={s} {gen}", flush=True)
+    g = O.uniform if gen == "u" else O.wide
+    A, B, C0 = g((m, k), 1), g((k, n), 2), O.uniform((m, n), 3)
+    dC = dev(C0)
+    info = phpc.dgemm_split_device(dev(A), dev(B), dC, slices=s, method=phpc.SPLIT_SLICES)
+    torch.cuda.synchronize()
+    assert info.slices == s and info.products == s * (s + 1) // 2
+    want = O.split_gemm(A, B, C0.copy(), s)
+    assert np.array_equal(bits(dC.cpu().numpy()), bits(want))
+
+
+def test_split_k_pieces_match_restatement():
+    """k = 20000 with 8 slices: level 9 spans 8 padded slices (160256 products), more
+    than one int32 accumulation can hold, so it runs in two pieces."""
+    import torch
+    m, k, n, s = 16, 20000, 24, 8
+    A, B = O.uniform((m, k), 4), O.uniform((k, n), 5)
+    dC = torch.zeros((m, n), dtype=torch.float64, device="cuda")
+    info = phpc.dgemm_split_device(dev(A), dev(B), dC, slices=s, method=phpc.SPLIT_SLICES)
+    torch.cuda.synchronize()
+    assert info.launches > s + 6  # at least one level was split
+    want = O.split_gemm(A, B, np.zeros((m, n)), s)
+    assert np.array_equal(bits(dC.cpu().numpy()), bits(want))
+
+
+def test_split_exact_on_small_integers():
+    import torch
+    rng = np.random.default_rng(9)
+    A = rng.integers(-127, 128, (70, 90)).astype(np.float64)
+    B = rng.integers(-127, 128, (90, 50)).astype(np.float64)
+    dC = torch.zeros((70, 50), dtype=torch.float64, device="cuda")
+    phpc.dgemm_split_device(dev(A), dev(B), dC, slices=1, method=phpc.SPLIT_SLICES)
+    torch.cuda.synchronize()
+    assert np.array_equal(dC.cpu().numpy(), A @ B)
+
+
+@pytest.mark.parametrize("gen,method", [("u", 0), ("w", 0), ("u", 1), ("w", 1)])
+def test_split_auto_slices_meet_fp64_target(gen, method):
+    import torch
+    N = 512
+    g = O.uniform if gen == "u" else O.wide
+    A, B = g((N, N), 11), g((N, N), 12)
+    dC = torch.zeros((N, N), dtype=torch.float64, device="cuda")
+    info = phpc.dgemm_split_device(dev(A), dev(B), dC, slices=0, target=1e-14, method=method)
+    torch.cuda.synchronize()
+    cw, _ = O.err_full(dC.cpu().numpy(), O.gemm_dd(A, B), A, B)
+    print(f"  {gen}: slices={info.slices} estimate={info.error_estimate:.2e} err_cw={cw:.2e}", flush=True)
+    assert cw <= 1e-14
+    assert info.error_estimate <= 1e-14
+
+
+def test_split_rejects_non_finite():
+    import torch
+    A = np.ones((8, 8))
+    A[3, 4] = np.inf
+    dC = torch.zeros((8, 8), dtype=torch.float64, device="cuda")
+    with pytest.raises(RuntimeError):
+        phpc.dgemm_split_device(dev(A), dev(np.ones((8, 8))), dC, slices=3)
+
+
+@pytest.mark.parametrize("dist,method", [(phpc.DIST_UNIFORM, 0), (phpc.DIST_WIDE, 0), (phpc.DIST_UNIFORM, 1),
+                                         (phpc.DIST_WIDE, 1)])
+def test_split_large_sampled_accuracy(dist, method):
+    import torch
+    N = 4096
+    dA = torch.empty((N, N), dtype=torch.float64, device="cuda")
+    dB = torch.empty_like(dA)
+    phpc.fill_device(dA, dist, 1)
+    phpc.fill_device(dB, dist, 2)
+    dC = torch.zeros_like(dA)
+    info = phpc.dgemm_split_device(dA, dB, dC, slices=0, target=1e-14, method=method)
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(4)
+    rows, cols = rng.integers(0, N, 2048), rng.integers(0, N, 2048)
+    cw, nw = phpc.check_sampled(dA, dB, dC, rows, cols)
+    print(f"  dist={dist} slices={info.slices} err_cw={cw:.2e}", flush=True)
+    assert cw <= 1e-14
+
+
+@pytest.mark.parametrize("m,k,n,L,gen", [(1, 1, 1, 3, "u"), (7, 9, 5, 6, "u"), (64, 64, 64, 14, "u"),
+                                         (200, 300, 130, 14, "w"), (257, 129, 255, 10, "u"),
+                                         (96, 1000, 80, 16, "w"), (33, 65, 300, 4, "u")])
+def test_crt_matches_restatement_bitwise(m, k, n, L, gen):
+    import torch
+    print(f"crt m={m} k={k} n={n} L={L} {gen}", flush=True)
+    g = O.uniform if gen == "u" else O.wide
+    A, B, C0 = g((m, k), 5), g((k, n), 6), O.uniform((m, n), 7)
+    dC = dev(C0)
+    info = phpc.dgemm_split_device(dev(A), dev(B), dC, slices=L, method=phpc.SPLIT_CRT)
+    torch.cuda.synchronize()
+    assert info.method == 1 and info.slices == L
+    want = O.crt_gemm(A, B, C0.copy(), info.int_bits, L)
+    assert np.array_equal(bits(dC.cpu().numpy()), bits(want))
+
+
+def test_crt_exact_on_integers():
+    """Integer inputs whose product fits the moduli come back exact."""
+    import torch
+    rng = np.random.default_rng(12)
+    A = rng.integers(-1000, 1001, (50, 70)).astype(np.float64)
+    B = rng.integers(-1000, 1001, (70, 40)).astype(np.float64)
+    dC = torch.zeros((50, 40), dtype=torch.float64, device="cuda")
+    info = phpc.dgemm_split_device(dev(A), dev(B), dC, slices=8, method=phpc.SPLIT_CRT)
+    torch.cuda.synchronize()
+    assert info.int_bits >= 10  # 1000 < 2^10 fits the integer width, so scaling is exact
+    assert np.array_equal(dC.cpu().numpy(), A @ B)

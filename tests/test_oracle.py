@@ -250,3 +250,100 @@ def test_err_full_planted():
     ab = np.abs(A) @ np.abs(B)
     assert cw == pytest.approx(1e-10 / ab[2, 3], rel=1e-6)
     assert nw == pytest.approx(1e-10 / np.abs(ref).max(), rel=1e-6)
+
+
+# ---------------- split-precision restatement ----------------
+
+def test_split_digits_reconstruct():
+    rng = np.random.default_rng(7)
+    for _ in range(200):
+        x = rng.standard_normal() * 2.0 ** rng.integers(-30, 30)
+        e = O.exp_of_max(abs(x))
+        for s in (1, 3, 7):
+            d = O.split_digits(x, e, s)
+            assert all(-127 <= t <= 127 for t in d)
+            recon = sum(Fraction(t) * Fraction(2) ** (e - 7 * (i + 1)) for i, t in enumerate(d))
+            assert abs(Fraction(x) - recon) < Fraction(2) ** (e - 7 * s)
+    # exact cases: an integer that fits one 7-bit digit; 2^-9 with e = 0 sits in slice 2
+    assert O.split_digits(100.0, 7, 3) == [100, 0, 0]
+    assert O.split_digits(2.0 ** -9, 0, 3) == [0, 32, 0]
+
+
+def test_split_exact_for_small_integers():
+    rng = np.random.default_rng(8)
+    A = rng.integers(-127, 128, (9, 40)).astype(np.float64)
+    B = rng.integers(-127, 128, (40, 11)).astype(np.float64)
+    C0 = rng.integers(-50, 50, (9, 11)).astype(np.float64)
+    got = O.split_gemm(A, B, C0.copy(), 1)
+    assert np.array_equal(got, A @ B + C0)
+
+
+def test_split_error_within_truncation_bound_and_decreasing():
+    m, k, n = 24, 300, 20
+    for gen in (O.uniform, O.wide):
+        A, B = gen((m, k), 1), gen((k, n), 2)
+        ref = O.gemm_dd(A, B)
+        eA = np.array([O.exp_of_max(np.abs(A[i]).max()) for i in range(m)])
+        eB = np.array([O.exp_of_max(np.abs(B[:, j]).max()) for j in range(n)])
+        ab = np.abs(A) @ np.abs(B)
+        prev = None
+        for s in (2, 3, 4, 5, 6, 7, 8):
+            C = O.split_gemm(A, B, np.zeros((m, n)), s)
+            # rigorous: each product loses < (s+1) 2^(-7s) 2^(e_i+f_j); plus a few roundings
+            bound = k * (s + 1) * 2.0 ** (-7 * s) * np.ldexp(1.0, eA[:, None] + eB[None, :]) + 8 * 2.0 ** -53 * ab
+            assert np.all(np.abs(C - ref) <= bound)
+            cw, _ = O.err_full(C, ref, A, B)
+            if prev is not None and prev > 1e-15:
+                assert cw < prev
+            prev = cw
+        assert prev <= 1e-15
+
+
+# ---------------- CRT restatement ----------------
+
+def test_crt_digits_reconstruct_exact_integers():
+    rng = np.random.default_rng(11)
+    for L in (2, 5, 14, 16, 20):
+        mods = O.MODULI[:L]
+        M = 1
+        for mm in mods:
+            M *= mm
+        for _ in range(50):
+            X = int(rng.integers(-(2 ** 62), 2 ** 62)) * int(rng.integers(0, 2 ** 40)) % M - M // 2
+            X = max(min(X, M // 2 - 1), -(M // 2) + 1)
+            res = []
+            for mm in mods:
+                r = X % mm
+                if r > (mm - 1) // 2:
+                    r -= mm
+                res.append(r)
+            v = O.crt_digits(res)
+            val, P = 0, 1
+            for t, mm in enumerate(mods):
+                val += v[t] * P
+                P *= mm
+            assert val == X
+
+
+def test_crt_gemm_within_integerisation_bound():
+    m, k, n = 20, 257, 18
+    for gen in (O.uniform, O.wide):
+        A, B = gen((m, k), 3), gen((k, n), 4)
+        ref = O.gemm_dd(A, B)
+        eA = np.array([O.exp_of_max(np.abs(A[i]).max()) for i in range(m)])
+        eB = np.array([O.exp_of_max(np.abs(B[:, j]).max()) for j in range(n)])
+        ab = np.abs(A) @ np.abs(B)
+        prev = None
+        for p, L in ((10, 4), (20, 7), (30, 9), (40, 12), (48, 14)):
+            C = O.crt_gemm(A, B, np.zeros((m, n)), p, L)
+            # |A - A'2^(p-e)| <= 2^(e-p-1) per entry (rounding to integers), likewise B
+            ea = np.ldexp(1.0, eA - p - 1)[:, None]
+            eb = np.ldexp(1.0, eB - p - 1)[None, :]
+            bound = (ea * (np.abs(B).sum(axis=0)[None, :]) + eb * (np.abs(A).sum(axis=1)[:, None])
+                     + k * ea * eb + 4 * 2.0 ** -53 * ab)
+            assert np.all(np.abs(C - ref) <= bound), (p, L)
+            cw, _ = O.err_full(C, ref, A, B)
+            if prev is not None and prev > 1e-15:
+                assert cw < prev
+            prev = cw
+        assert prev <= 1e-14
