@@ -217,17 +217,20 @@ struct LevelArgs {
 };
 
 // x mod m in the symmetric range (m odd: |r| <= (m-1)/2; m = 256: [-128, 127]) for
-// integer-valued doubles |x| < 2^52, by an exact floor division in double.
-__device__ __forceinline__ int smod_d(double x, int m) {
+// integer-valued doubles |x| < 2^40, with rm = 1/m rounded: the quotient estimate
+// floor(x * rm) is off by at most one (|x/m| * 2^-52 < 2^-19), so one correction step
+// and an exact remainder x - q*m give the residue without a division.
+__device__ __forceinline__ int smod_r(double x, int m, double rm) {
   const double md = (double)m;
-  double q = floor(x / md);
-  double r = x - q * md;  // exact: |q * m| < 2^53
+  const double q = floor(x * rm);
+  double r = x - q * md;  // exact: |q * m| < 2^41
   if (r < 0.0) r += md;
   if (r >= md) r -= md;
   int ri = (int)r;
   if (ri > (m - 1) / 2) ri -= m;
   return ri;
 }
+__device__ __forceinline__ int smod_d(double x, int m) { return smod_r(x, m, 1.0 / (double)m); }
 
 // LDS layout per stage: A then B, BM (BN) rows of BK k-bytes each, in 16-byte chunks.
 // Chunk c of row r sits at slot c ^ sw(r), sw(r) = (r * CPR / 16) mod CPR (CPR chunks per
@@ -378,6 +381,7 @@ __device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
     const int mod_m = uni(args.mod_m);
     if (mod_m > 0) {
       const int firstr = uni(args.first);
+      const double rmod = 1.0 / (double)mod_m;
       int8_t *resg = uni(args.res);
       const long ldres = uni(args.ldres);
 #pragma unroll
@@ -393,7 +397,7 @@ __device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
             int8_t *pr = resg + (long)row * ldres + col;
             double v = (double)acc[i][j][r];
             if (!firstr) v += (double)*pr;
-            *pr = (int8_t)smod_d(v, mod_m);
+            *pr = (int8_t)smod_r(v, mod_m, rmod);
           }
         }
       continue;
@@ -519,14 +523,8 @@ struct CrtConsts {
   int c48[kMaxMods];           // 2^48 mod m
   int inv[kMaxMods];           // (m_1 ... m_{t-1})^-1 mod m_t
   int pmod[kMaxMods][kMaxMods];  // pmod[s][t] = (m_1 ... m_{s-1}) mod m_t, s < t
+  double rm[kMaxMods];           // 1 / m, rounded
 };
-
-// residue of an integer-valued double |x| < 2^72 modulo m, symmetric range
-__device__ __forceinline__ int8_t residue_of(double x, const CrtConsts &cc, int t, double h, double mid, double lo) {
-  (void)x;
-  const double sum = h * (double)cc.c48[t] + mid * (double)cc.c24[t] + lo;  // < 2^34, exact
-  return (int8_t)smod_d(sum, cc.m[t]);
-}
 
 __device__ __forceinline__ void split24(double ax, double &h, double &mid, double &lo) {
   h = trunc(ldexp(ax, -48));
@@ -535,7 +533,30 @@ __device__ __forceinline__ void split24(double ax, double &h, double &mid, doubl
   lo = rest - ldexp(mid, 24);
 }
 
-// A8 row i: residue plane t at bytes [t*kp, t*kp + k), zero to kp
+// Symmetric residues of one integer-valued double x (|x| <= 2^71) for the first L
+// moduli, packed: byte t of the result array = residue modulo m_t.
+template <int L>
+__device__ __forceinline__ void residues_of(double x, const CrtConsts &cc, int (&r)[L]) {
+  const double ax = fabs(x);
+  double h, mid, lo;
+  split24(ax, h, mid, lo);
+#pragma unroll
+  for (int t = 0; t < L; ++t) {
+    const double sum = h * (double)cc.c48[t] + mid * (double)cc.c24[t] + lo;  // < 2^34, exact
+    int v = smod_r(sum, cc.m[t], cc.rm[t]);
+    if (x < 0.0) v = -v;
+    if (v == 128) v = -128;  // m = 256: keep [-128, 127]
+    r[t] = v;
+  }
+}
+
+__device__ __forceinline__ uint32_t pack4(int a, int b, int c, int d) {
+  return (uint32_t)(a & 0xff) | ((uint32_t)(b & 0xff) << 8) | ((uint32_t)(c & 0xff) << 16) | ((uint32_t)(d & 0xff) << 24);
+}
+
+// A8 row i: residue plane t at bytes [t*kp, t*kp + k), zero to kp. Each thread handles
+// four consecutive k and stores one 32-bit word per plane.
+template <int L>
 __global__ __launch_bounds__(256) void residue_rows_kernel(const double *__restrict__ A, long lda, int k, int p,
                                                            long kp, const int *__restrict__ eA, CrtConsts cc,
                                                            int8_t *__restrict__ A8, long ld8) {
@@ -543,21 +564,23 @@ __global__ __launch_bounds__(256) void residue_rows_kernel(const double *__restr
   const double *row = A + (long)i * lda;
   const int sh = p - eA[i];
   int8_t *out = A8 + (long)i * ld8;
-  for (long l = threadIdx.x; l < kp; l += 256) {
-    const double x = l < k ? rint(ldexp(row[l], sh)) : 0.0;
-    const double ax = fabs(x);
-    double h, mid, lo;
-    split24(ax, h, mid, lo);
-    for (int t = 0; t < cc.L; ++t) {
-      int r = residue_of(ax, cc, t, h, mid, lo);
-      if (x < 0.0) r = -r;
-      if (r == 128) r = -128;  // m = 256: keep [-128, 127]
-      out[(long)t * kp + l] = (int8_t)r;
+  for (long l0 = (long)threadIdx.x * 4; l0 < kp; l0 += 1024) {
+    int r[4][L];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const long l = l0 + q;
+      const double x = l < k ? rint(ldexp(row[l], sh)) : 0.0;
+      residues_of<L>(x, cc, r[q]);
     }
+#pragma unroll
+    for (int t = 0; t < L; ++t)
+      *reinterpret_cast<uint32_t *>(out + (long)t * kp + l0) = pack4(r[0][t], r[1][t], r[2][t], r[3][t]);
   }
 }
 
-// B8 row j (column j of B): residue plane t at bytes [t*kp, t*kp + k)
+// B8 row j (column j of B): residue plane t at bytes [t*kp, t*kp + k). 64 (k) x 64 (j)
+// tiles pass through LDS; each thread then takes four consecutive k of one column.
+template <int L>
 __global__ __launch_bounds__(256) void residue_cols_kernel(const double *__restrict__ B, long ldb, int k, int n, int p,
                                                            long kp, const int *__restrict__ eB, CrtConsts cc,
                                                            int8_t *__restrict__ B8, long ld8) {
@@ -570,21 +593,18 @@ __global__ __launch_bounds__(256) void residue_cols_kernel(const double *__restr
     tile[r][tx] = (kk < k && jj < n) ? B[(long)kk * ldb + jj] : 0.0;
   }
   __syncthreads();
-  for (int c = ty; c < 64; c += 4) {
+  const int kq = (threadIdx.x & 15) * 4;  // four consecutive k
+  for (int c = threadIdx.x >> 4; c < 64; c += 16) {
     const int jj = j0 + c;
     if (jj >= n) continue;
-    const int kk = k0 + tx;
-    const double x = rint(ldexp(tile[tx][c], p - eB[jj]));
-    const double ax = fabs(x);
-    double h, mid, lo;
-    split24(ax, h, mid, lo);
-    int8_t *out = B8 + (long)jj * ld8;
-    for (int t = 0; t < cc.L; ++t) {
-      int r = residue_of(ax, cc, t, h, mid, lo);
-      if (x < 0.0) r = -r;
-      if (r == 128) r = -128;
-      out[(long)t * kp + kk] = (int8_t)r;
-    }
+    const int sh = p - eB[jj];
+    int r[4][L];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) residues_of<L>(rint(ldexp(tile[kq + q][c], sh)), cc, r[q]);
+    int8_t *out = B8 + (long)jj * ld8 + k0 + kq;
+#pragma unroll
+    for (int t = 0; t < L; ++t)
+      *reinterpret_cast<uint32_t *>(out + (long)t * kp) = pack4(r[0][t], r[1][t], r[2][t], r[3][t]);
   }
 }
 
@@ -592,47 +612,120 @@ __global__ __launch_bounds__(256) void residue_cols_kernel(const double *__restr
 // C[i,j] += 2^(e_i + f_j - 2p) * C'[i,j], C' from its residues (planes of m x n bytes):
 // Garner's mixed-radix digits v_t in symmetric ranges (exact small-integer arithmetic),
 // then C' = v_1 + m_1 (v_2 + m_2 (v_3 + ...)) evaluated in double-double.
+template <int L>
+__device__ __forceinline__ double crt_value(const int (&r)[L], const CrtConsts &cc) {
+  int v[L];
+#pragma unroll
+  for (int t = 0; t < L; ++t) {
+    int y = 0;
+#pragma unroll
+    for (int s2 = 0; s2 < t; ++s2) y += v[s2] * cc.pmod[s2][t];  // |y| < 20 * 128 * 256
+    // v_t = (r_t - y) * inv_t  mod m_t (|(r - y) * inv| < 2^29: exact in double)
+    v[t] = smod_r((double)(r[t] - y) * (double)cc.inv[t], cc.m[t], cc.rm[t]);
+  }
+  double hi = (double)v[L - 1], lo = 0.0;
+#pragma unroll
+  for (int t = L - 2; t >= 0; --t) {
+    // (hi, lo) = (hi, lo) * m_t + v_t
+    const double mt = (double)cc.m[t];
+    const double ph = hi * mt;
+    const double pe = fma(hi, mt, -ph);
+    const double pl = lo * mt + pe;
+    const double sh = ph + (double)v[t];
+    const double bb = sh - ph;
+    const double se = (ph - (sh - bb)) + ((double)v[t] - bb);
+    const double tl = pl + se;
+    hi = sh + tl;
+    lo = tl - (hi - sh);
+  }
+  return hi + lo;
+}
+
+template <int L>
 __global__ __launch_bounds__(256) void crt_reconstruct_kernel(const int8_t *__restrict__ res, long plane, int m, int n,
                                                               CrtConsts cc, int p, const int *__restrict__ eA,
                                                               const int *__restrict__ eB, double *__restrict__ C,
                                                               long ldc) {
+  // four consecutive outputs of one row per thread (n is a multiple of 4 on this path)
+  const long total4 = (long)m * n / 4;
+  for (long e4 = (long)blockIdx.x * blockDim.x + threadIdx.x; e4 < total4; e4 += (long)gridDim.x * blockDim.x) {
+    const long e = e4 * 4;
+    uint32_t w[L];
+#pragma unroll
+    for (int t = 0; t < L; ++t) w[t] = *reinterpret_cast<const uint32_t *>(res + (long)t * plane + e);
+    const long i = e / n, j = e % n;
+    double *pc = C + i * ldc + j;
+    const int ei = eA[i] - 2 * p;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int r[L];
+#pragma unroll
+      for (int t = 0; t < L; ++t) r[t] = (int)(int8_t)(w[t] >> (8 * q));
+      const double cp = crt_value<L>(r, cc);
+      pc[q] = pc[q] + ldexp(cp, ei + eB[j + q]);
+    }
+  }
+}
+
+// general shapes: one output per thread
+template <int L>
+__global__ __launch_bounds__(256) void crt_reconstruct1_kernel(const int8_t *__restrict__ res, long plane, int m, int n,
+                                                               CrtConsts cc, int p, const int *__restrict__ eA,
+                                                               const int *__restrict__ eB, double *__restrict__ C,
+                                                               long ldc) {
   const long total = (long)m * n;
   for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
-    int v[kMaxMods];
-    for (int t = 0; t < cc.L; ++t) {
-      const int r = res[(long)t * plane + e];
-      int y = 0;
-      for (int s2 = 0; s2 < t; ++s2) y += v[s2] * cc.pmod[s2][t];  // |y| < 20 * 128 * 256
-      // v_t = (r - y) * inv_t  mod m_t, symmetric
-      const int mt = cc.m[t];
-      int d = (r - y) % mt;
-      if (d < 0) d += mt;
-      long long w = (long long)d * cc.inv[t] % mt;
-      int vt = (int)w;
-      if (vt > (mt - 1) / 2) vt -= mt;
-      v[t] = vt;
-    }
-    double hi = (double)v[cc.L - 1], lo = 0.0;
-    for (int t = cc.L - 2; t >= 0; --t) {
-      // (hi, lo) = (hi, lo) * m_t + v_t
-      const double mt = (double)cc.m[t];
-      const double ph = hi * mt;
-      const double pe = fma(hi, mt, -ph);
-      const double pl = lo * mt + pe;
-      const double sh = ph + (double)v[t];
-      const double bb = sh - ph;
-      const double se = (ph - (sh - bb)) + ((double)v[t] - bb);
-      const double tl = pl + se;
-      hi = sh + tl;
-      lo = tl - (hi - sh);
-    }
-    const double cp = hi + lo;
+    int r[L];
+#pragma unroll
+    for (int t = 0; t < L; ++t) r[t] = res[(long)t * plane + e];
+    const double cp = crt_value<L>(r, cc);
     const long i = e / n, j = e % n;
     double *pc = C + i * ldc + j;
     *pc = *pc + ldexp(cp, eA[i] + eB[j] - 2 * p);
   }
 }
 #pragma clang fp contract(on)
+
+// launches the kernels above for a runtime modulus count
+struct CrtLaunch {
+  template <int L>
+  static int run(hipStream_t s, const double *dA, long lda, const double *dB, long ldb, int m, int k, int n, int p,
+                 long kp, const int *eA, const int *eB, const CrtConsts &cc, int8_t *A8, int8_t *B8, long ld8,
+                 int stage, const int8_t *R8, double *dC, long ldc) {
+    if (stage == 0) {
+      hipLaunchKernelGGL(residue_rows_kernel<L>, dim3(m), dim3(256), 0, s, dA, lda, k, p, kp, eA, cc, A8, ld8);
+      PHPC_HIP(hipGetLastError());
+      hipLaunchKernelGGL(residue_cols_kernel<L>, dim3((unsigned)((kp + 63) / 64), (unsigned)((n + 63) / 64)), dim3(256),
+                         0, s, dB, ldb, k, n, p, kp, eB, cc, B8, ld8);
+      PHPC_HIP(hipGetLastError());
+    } else {
+      const long total = (long)m * n;
+      if (n % 4 == 0) {
+        long blocks = std::min<long>((total / 4 + 255) / 256, 65536);
+        hipLaunchKernelGGL(crt_reconstruct_kernel<L>, dim3((unsigned)blocks), dim3(256), 0, s, R8, total, m, n, cc, p,
+                           eA, eB, dC, ldc);
+      } else {
+        long blocks = std::min<long>((total + 255) / 256, 65536);
+        hipLaunchKernelGGL(crt_reconstruct1_kernel<L>, dim3((unsigned)blocks), dim3(256), 0, s, R8, total, m, n, cc,
+                           p, eA, eB, dC, ldc);
+      }
+      PHPC_HIP(hipGetLastError());
+    }
+    return PHPC_OK;
+  }
+};
+
+template <int L>
+int crt_dispatch_t(int Lrt, hipStream_t s, const double *dA, long lda, const double *dB, long ldb, int m, int k, int n,
+                   int p, long kp, const int *eA, const int *eB, const CrtConsts &cc, int8_t *A8, int8_t *B8, long ld8,
+                   int stage, const int8_t *R8, double *dC, long ldc) {
+  if (Lrt == L) return CrtLaunch::run<L>(s, dA, lda, dB, ldb, m, k, n, p, kp, eA, eB, cc, A8, B8, ld8, stage, R8, dC, ldc);
+  if constexpr (L < 20) {
+    return crt_dispatch_t<L + 1>(Lrt, s, dA, lda, dB, ldb, m, k, n, p, kp, eA, eB, cc, A8, B8, ld8, stage, R8, dC, ldc);
+  } else {
+    return phpc_set_error(PHPC_ERR_ARG, "modulus count %d out of range", Lrt);
+  }
+}
 
 int modinv(int a, int m) {
   // extended Euclid, a and m coprime
@@ -665,6 +758,7 @@ CrtConsts make_crt(int L) {
       prod = prod * kModuli[s2] % m;
     }
     c.inv[t] = t == 0 ? 1 : modinv((int)prod, m);
+    c.rm[t] = 1.0 / (double)m;
   }
   return c;
 }
@@ -781,11 +875,11 @@ extern "C" int phpc_dgemm_split_device(const double *dA, long lda, const double 
     int8_t *B8 = (int8_t *)phpc_scratch((size_t)n * ld8, 9);
     int8_t *R8 = (int8_t *)phpc_scratch((size_t)L * m * n, 12);
     if (!A8 || !B8 || !R8) return phpc_set_error(PHPC_ERR_MEM, "CRT scratch allocation failed");
-    hipLaunchKernelGGL(residue_rows_kernel, dim3(m), dim3(256), 0, s, dA, lda, k, pbits, kp, eA, cc, A8, ld8);
-    PHPC_HIP(hipGetLastError());
-    hipLaunchKernelGGL(residue_cols_kernel, dim3((unsigned)((kp + 63) / 64), (unsigned)((n + 63) / 64)), dim3(256),
-                       0, s, dB, ldb, k, n, pbits, kp, eB, cc, B8, ld8);
-    PHPC_HIP(hipGetLastError());
+    {
+      int rc = crt_dispatch_t<2>(L, s, dA, lda, dB, ldb, m, k, n, pbits, kp, eA, eB, cc, A8, B8, ld8, 0, nullptr,
+                                 nullptr, 0);
+      if (rc) return rc;
+    }
     long tiles256 = (long)((m + 255) / 256) * ((n + 255) / 256);
     const int cfg = tiles256 >= (long)phpc_device().cus / 2 ? g_i8_big : 2;
     const long piece = std::min<long>(kp, kMaxExactTermsCrt);
@@ -811,11 +905,10 @@ extern "C" int phpc_dgemm_split_device(const double *dA, long lda, const double 
         ++launches;
       }
     }
-    const long total = (long)m * n;
-    long blocks = std::min<long>((total + 255) / 256, 65536);
-    hipLaunchKernelGGL(crt_reconstruct_kernel, dim3((unsigned)blocks), dim3(256), 0, s, R8, (long)m * n, m, n, cc,
-                       pbits, eA, eB, dC, ldc);
-    PHPC_HIP(hipGetLastError());
+    {
+      int rc = crt_dispatch_t<2>(L, s, dA, lda, dB, ldb, m, k, n, pbits, kp, eA, eB, cc, A8, B8, ld8, 1, R8, dC, ldc);
+      if (rc) return rc;
+    }
     if (info) {
       info->slices = L;
       info->products = L;
