@@ -1,0 +1,294 @@
+// FP32 grouped GEMM on gfx950 matrix cores (v_mfma_f32_32x32x2f32): C += A*B for a list
+// of independent row-major FP32 GEMMs of any shapes, one persistent launch.
+//
+// Same structure as the FP64 kernel: a BM x BN macro tile per workgroup, 4 waves in a
+// 2 x 2 arrangement, operands staged HBM -> registers (raw buffer loads, out-of-range
+// lanes read zero) -> LDS (A transposed, At[k][m]; B as Bs[k][n]) in 16-deep slabs with
+// double buffering, and a persistent workgroup loop over the flat tile list of all GEMMs.
+// MFMA 32x32x2: lane l holds A[row l%32][k l/32] and B[k l/32][col l%32]; the 16
+// accumulators of lane l are rows 8(r/4) + 4(l/32) + r%4, column l%32.
+#include "phpc_internal.h"
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+namespace {
+
+typedef float f16v __attribute__((ext_vector_type(16)));
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+typedef float __attribute__((address_space(1))) gfloat;
+
+constexpr uint32_t kOOB = 0x80000000u;
+constexpr int BK = 16;
+
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ long uni(long v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+  return (long)(((uint64_t)hi << 32) | lo);
+}
+template <typename T>
+__device__ __forceinline__ T *uni(T *p) {
+  return reinterpret_cast<T *>(uni((long)reinterpret_cast<uintptr_t>(p)));
+}
+__device__ __forceinline__ rsrc_t make_rsrc(const void *base, long bytes) {
+  const uint32_t nb = bytes > 0x7fffffffL ? 0x7fffffffu : (uint32_t)(bytes < 0 ? 0 : bytes);
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)nb, 0x00020000);
+}
+
+struct F32Batch {
+  const F32Desc *descs;
+  int ndesc;
+  int total_tiles;
+  F32Desc d0;
+};
+
+__device__ __forceinline__ F32Desc uniform_desc(const F32Batch &b, int t) {
+  const F32Desc *d = &b.d0;
+  if (b.ndesc > 1) {
+    int lo = 0, hi = b.ndesc - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (b.descs[mid].tile_start <= t)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    d = &b.descs[lo];
+  }
+  F32Desc u;
+  u.A = uni(d->A);
+  u.B = uni(d->B);
+  u.C = uni(d->C);
+  u.lda = uni(d->lda);
+  u.ldb = uni(d->ldb);
+  u.ldc = uni(d->ldc);
+  u.m = uni(d->m);
+  u.n = uni(d->n);
+  u.k = uni(d->k);
+  u.tiles_n = uni(d->tiles_n);
+  u.tile_start = uni(d->tile_start);
+  return u;
+}
+
+template <int BM, int BN, int VEC>
+__global__ __launch_bounds__(256) void f32_gemm_kernel(F32Batch batch) {
+  constexpr int WTM = BM / 2, WTN = BN / 2;
+  constexpr int TM = WTM / 32, TN = WTN / 32;
+  constexpr int APITCH = BM + 4, BPITCH = BN + 4;
+  constexpr int STAGE = BK * APITCH + BK * BPITCH;
+  constexpr int CA = BM * BK / VEC / 256, CB = BN * BK / VEC / 256;
+  static_assert(CA * VEC * 256 == BM * BK && CB * VEC * 256 == BN * BK, "slab split");
+  static_assert(TM >= 1 && TN >= 1, "wave tile");
+  __shared__ float lds[2 * STAGE];
+
+  const int tid = threadIdx.x;
+  const int wave = uni(tid >> 6);
+  const int lane = tid & 63;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int G = gridDim.x, bid = blockIdx.x;
+  const int slot = uni((G % 8) == 0 ? (bid % 8) * (G / 8) + bid / 8 : bid);
+
+  for (int round = 0;; ++round) {
+    const int t = round * G + slot;
+    if (t >= batch.total_tiles) break;
+    const F32Desc d = uniform_desc(batch, uni(t));
+    const int lt = t - d.tile_start;
+    const int row0 = uni((lt / d.tiles_n) * BM);
+    const int col0 = uni((lt % d.tiles_n) * BN);
+    const int m_rem = d.m - row0, n_rem = d.n - col0;
+    const long lda = d.lda, ldb = d.ldb, ldc = d.ldc;
+    const float *Ag = d.A + (long)row0 * lda;
+    const float *Bg = d.B + col0;
+    gfloat *Cg = (gfloat *)(d.C + (long)row0 * ldc + col0);
+    const rsrc_t rA = make_rsrc(Ag, (long)min(m_rem, BM) * lda * 4);
+
+    f16v acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) acc[i][j] = f16v{};
+
+    float ra[CA][VEC], rb[CB][VEC];
+    auto load = [&](int kb) {
+      const int krem = d.k - kb;
+#pragma unroll
+      for (int i = 0; i < CA; ++i) {
+        const int q = tid + i * 256;
+        const int row = q / (BK / VEC), kc = (q % (BK / VEC)) * VEC;
+        const uint32_t off = (row < m_rem && kc < krem) ? (uint32_t)(((long)row * lda + kb + kc) * 4) : kOOB;
+        if constexpr (VEC == 4) {
+          const auto v = __builtin_amdgcn_raw_buffer_load_b128(rA, off, 0, 0);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) ra[i][e] = __builtin_bit_cast(float, v[e]);
+        } else {
+          ra[i][0] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rA, off, 0, 0));
+        }
+      }
+      const rsrc_t rB = make_rsrc(Bg + (long)kb * ldb, (long)krem * ldb * 4);
+#pragma unroll
+      for (int i = 0; i < CB; ++i) {
+        const int q = tid + i * 256;
+        const int kr = q / (BN / VEC), cc = (q % (BN / VEC)) * VEC;
+        const uint32_t off = (kr < krem && cc < n_rem) ? (uint32_t)(((long)kr * ldb + cc) * 4) : kOOB;
+        if constexpr (VEC == 4) {
+          const auto v = __builtin_amdgcn_raw_buffer_load_b128(rB, off, 0, 0);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) rb[i][e] = __builtin_bit_cast(float, v[e]);
+        } else {
+          rb[i][0] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rB, off, 0, 0));
+        }
+      }
+    };
+    auto store = [&](int buf) {
+      float *At = lds + buf * STAGE;
+      float *Bs = At + BK * APITCH;
+#pragma unroll
+      for (int i = 0; i < CA; ++i) {
+        const int q = tid + i * 256;
+        const int row = q / (BK / VEC), kc = (q % (BK / VEC)) * VEC;
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) At[(kc + e) * APITCH + row] = ra[i][e];
+      }
+#pragma unroll
+      for (int i = 0; i < CB; ++i) {
+        const int q = tid + i * 256;
+        const int kr = q / (BN / VEC), cc = (q % (BN / VEC)) * VEC;
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) Bs[kr * BPITCH + cc + e] = rb[i][e];
+      }
+    };
+
+    __syncthreads();
+    const int nslab = (d.k + BK - 1) / BK;
+    load(0);
+    store(0);
+    __syncthreads();
+    for (int sidx = 0; sidx < nslab; ++sidx) {
+      const int buf = sidx & 1;
+      if (sidx + 1 < nslab) load((sidx + 1) * BK);
+      const float *At = lds + buf * STAGE;
+      const float *Bs = At + BK * APITCH;
+#pragma unroll
+      for (int kk = 0; kk < BK / 2; ++kk) {
+        const int k = kk * 2 + (lane >> 5);
+        float a[TM], b[TN];
+#pragma unroll
+        for (int i = 0; i < TM; ++i) a[i] = At[k * APITCH + wm * WTM + i * 32 + (lane & 31)];
+#pragma unroll
+        for (int j = 0; j < TN; ++j) b[j] = Bs[k * BPITCH + wn * WTN + j * 32 + (lane & 31)];
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i], b[j], acc[i][j], 0, 0, 0);
+      }
+      if (sidx + 1 < nslab) {
+        store(buf ^ 1);
+        __syncthreads();
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const int cc = wn * WTN + j * 32 + (lane & 31);
+        if (cc >= n_rem) continue;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int rr = wm * WTM + i * 32 + 8 * (r / 4) + 4 * (lane >> 5) + (r % 4);
+          if (rr < m_rem) {
+            gfloat *pc = Cg + (long)rr * ldc + cc;
+            *pc = *pc + acc[i][j][r];
+          }
+        }
+      }
+  }
+}
+
+typedef void (*F32Fn)(F32Batch);
+struct F32Cfg {
+  int bm, bn;
+  F32Fn fn[2];  // [vec4]
+};
+const F32Cfg kF32[] = {
+    {64, 64, {f32_gemm_kernel<64, 64, 1>, f32_gemm_kernel<64, 64, 4>}},
+    {128, 64, {f32_gemm_kernel<128, 64, 1>, f32_gemm_kernel<128, 64, 4>}},
+    {128, 128, {f32_gemm_kernel<128, 128, 1>, f32_gemm_kernel<128, 128, 4>}},
+};
+constexpr int kNumF32 = sizeof(kF32) / sizeof(kF32[0]);
+
+std::mutex g_mu;
+bool g_ready = false;
+int g_occ[kNumF32][2];
+
+int occ(int c, int v) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_ready) {
+    for (int i = 0; i < kNumF32; ++i)
+      for (int j = 0; j < 2; ++j) {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void *>(kF32[i].fn[j]), 256, 0) !=
+                hipSuccess ||
+            n < 1)
+          n = 1;
+        g_occ[i][j] = n;
+      }
+    g_ready = true;
+  }
+  return g_occ[c][v];
+}
+
+bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+}  // namespace
+
+int phpc_f32_num_configs() { return kNumF32; }
+
+void phpc_f32_config_tile(int id, int *bm, int *bn) {
+  *bm = kF32[id].bm;
+  *bn = kF32[id].bn;
+}
+
+// Fills tiles_n / tile_start of descs for config `id`; returns the total tile count.
+int phpc_f32_plan_tiles(F32Desc *descs, int ndesc, int id) {
+  int start = 0;
+  for (int i = 0; i < ndesc; ++i) {
+    F32Desc &d = descs[i];
+    d.tiles_n = (d.n + kF32[id].bn - 1) / kF32[id].bn;
+    d.tile_start = start;
+    start += ((d.m + kF32[id].bm - 1) / kF32[id].bm) * d.tiles_n;
+  }
+  return start;
+}
+
+bool phpc_f32_vec4_ok(const F32Desc *descs, int ndesc) {
+  for (int i = 0; i < ndesc; ++i) {
+    const F32Desc &d = descs[i];
+    if (!aligned16(d.A) || !aligned16(d.B) || d.lda % 4 || d.ldb % 4 || d.k % 4 || d.n % 4) return false;
+  }
+  return true;
+}
+
+// Launch the grouped kernel over device-resident descriptors (planned for config id).
+int phpc_f32_launch(const F32Desc *descs_dev, const F32Desc *descs_host, int ndesc, int total_tiles, int id, bool vec4,
+                    int grid, hipStream_t stream) {
+  if (ndesc <= 0 || total_tiles <= 0) return PHPC_OK;
+  for (int i = 0; i < ndesc; ++i) {
+    const F32Desc &d = descs_host[i];
+    if ((long)kF32[id].bm * d.lda * 4 >= 0x7fffffffL || 16L * d.ldb * 4 >= 0x7fffffffL)
+      return phpc_set_error(PHPC_ERR_ARG, "leading dimension too large");
+  }
+  F32Batch b{};
+  b.descs = descs_dev;
+  b.ndesc = ndesc;
+  b.total_tiles = total_tiles;
+  b.d0 = descs_host[0];
+  int G = grid > 0 ? grid : occ(id, vec4) * phpc_device().cus;
+  if (G > total_tiles) G = total_tiles;
+  if (G < 1) G = 1;
+  hipLaunchKernelGGL(kF32[id].fn[vec4 ? 1 : 0], dim3(G), dim3(256), 0, stream, b);
+  PHPC_HIP(hipGetLastError());
+  return PHPC_OK;
+}

@@ -847,3 +847,17 @@ extern "C" int phpc_set_tuning(int key, int value) {
   }
   return phpc_set_error(PHPC_OK, nullptr);
 }
+
+int phpc_f64_plan_tiles(GemmDesc *descs, int ndesc, int id) {
+  const ConfigEntry &ce = kConfigs[id];
+  int start = 0;
+  for (int i = 0; i < ndesc; ++i) {
+    GemmDesc &d = descs[i];
+    d.tiles_n = (d.n + ce.bn - 1) / ce.bn;
+    d.tile_start = start;
+    start += ((d.m + ce.bm - 1) / ce.bm) * d.tiles_n;
+  }
+  return start;
+}
+
+bool phpc_f64_config_panels(int id) { return id >= 0 && id < kNumConfigs && kConfigs[id].panels; }

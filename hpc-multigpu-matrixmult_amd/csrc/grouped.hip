@@ -1,0 +1,258 @@
+// Grouped GEMM plans: device-resident item lists, autotuned tile choice, hipGraph replay.
+#include "phpc_grouped.h"
+#include "phpc_internal.h"
+
+#include <chrono>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+struct phpc_plan {
+  int dtype = PHPC_F64;
+  int count = 0;
+  std::vector<GemmDesc> d64;
+  std::vector<F32Desc> d32;
+  void *descs_dev = nullptr;
+  int config = 0;
+  bool vec = false;
+  int total_tiles = 0;
+  int grid = 0;
+  double tuned_ms = 0.0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t graph_stream = nullptr;
+};
+
+namespace {
+
+std::mutex g_tune_mu;
+std::map<std::string, int> g_tune_cache;  // shape signature -> config
+
+std::string signature(int dtype, const phpc_gemm_item *it, int n) {
+  std::string s = std::to_string(dtype) + ":";
+  for (int i = 0; i < n; ++i) {
+    s += std::to_string(it[i].m) + "x" + std::to_string(it[i].n) + "x" + std::to_string(it[i].k) + "p" +
+         std::to_string(it[i].panel) + "," + std::to_string(it[i].lda % 4) + std::to_string(it[i].ldb % 4) + ";";
+  }
+  return s;
+}
+
+// (re)plan tiles for config id and upload the descriptors
+int plan_upload(phpc_plan *p, int id, hipStream_t s) {
+  size_t bytes;
+  const void *src;
+  if (p->dtype == PHPC_F64) {
+    p->total_tiles = phpc_f64_plan_tiles(p->d64.data(), p->count, id);
+    bytes = sizeof(GemmDesc) * p->count;
+    src = p->d64.data();
+  } else {
+    p->total_tiles = phpc_f32_plan_tiles(p->d32.data(), p->count, id);
+    p->vec = phpc_f32_vec4_ok(p->d32.data(), p->count);
+    bytes = sizeof(F32Desc) * p->count;
+    src = p->d32.data();
+  }
+  p->config = id;
+  PHPC_HIP(hipMemcpyAsync(p->descs_dev, src, bytes, hipMemcpyHostToDevice, s));
+  PHPC_HIP(hipStreamSynchronize(s));
+  return PHPC_OK;
+}
+
+int launch(phpc_plan *p, hipStream_t s) {
+  if (p->count == 0 || p->total_tiles == 0) return PHPC_OK;
+  if (p->dtype == PHPC_F64)
+    return phpc_f64_grouped_dev((const GemmDesc *)p->descs_dev, p->d64.data(), p->count, p->grid, p->config, s);
+  return phpc_f32_launch((const F32Desc *)p->descs_dev, p->d32.data(), p->count, p->total_tiles, p->config, p->vec,
+                         p->grid, s);
+}
+
+}  // namespace
+
+extern "C" int phpc_plan_create(int dtype, const phpc_gemm_item *items, int count, int flags, void *stream_,
+                                phpc_plan_t *out) {
+  if (!out) return phpc_set_error(PHPC_ERR_ARG, "null plan pointer");
+  *out = nullptr;
+  if (phpc_init()) return phpc_last_error();
+  if ((dtype != PHPC_F64 && dtype != PHPC_F32) || count < 0 || (count > 0 && !items))
+    return phpc_set_error(PHPC_ERR_ARG, "bad plan arguments");
+  hipStream_t s = (hipStream_t)stream_;
+  phpc_plan *p = new phpc_plan();
+  p->dtype = dtype;
+  std::vector<phpc_gemm_item> live;
+  for (int i = 0; i < count; ++i) {
+    const phpc_gemm_item &it = items[i];
+    if (it.m < 0 || it.n < 0 || it.k < 0 || (it.m && it.k && (!it.A || it.lda < it.k)) ||
+        (it.k && it.n && (!it.B || it.ldb < it.n)) || (it.m && it.n && (!it.C || it.ldc < it.n)) ||
+        (dtype == PHPC_F32 && it.panel > 0 && it.panel < it.k)) {
+      delete p;
+      return phpc_set_error(PHPC_ERR_ARG, "bad GEMM item %d", i);
+    }
+    if (it.m == 0 || it.n == 0 || it.k == 0) continue;
+    live.push_back(it);
+  }
+  p->count = (int)live.size();
+  bool mp = false;
+  for (const auto &it : live) {
+    if (dtype == PHPC_F64) {
+      GemmDesc d{};
+      d.A = (const double *)it.A;
+      d.B = (const double *)it.B;
+      d.C = (double *)it.C;
+      d.lda = it.lda;
+      d.ldb = it.ldb;
+      d.ldc = it.ldc;
+      d.m = it.m;
+      d.n = it.n;
+      d.k = it.k;
+      d.panel = it.panel > 0 && it.panel < it.k ? it.panel : 0;
+      if (d.panel) mp = true;
+      p->d64.push_back(d);
+    } else {
+      F32Desc d{};
+      d.A = (const float *)it.A;
+      d.B = (const float *)it.B;
+      d.C = (float *)it.C;
+      d.lda = it.lda;
+      d.ldb = it.ldb;
+      d.ldc = it.ldc;
+      d.m = it.m;
+      d.n = it.n;
+      d.k = it.k;
+      p->d32.push_back(d);
+    }
+  }
+  if (p->count == 0) {
+    *out = p;
+    return phpc_set_error(PHPC_OK, nullptr);
+  }
+  const size_t bytes = dtype == PHPC_F64 ? sizeof(GemmDesc) * p->count : sizeof(F32Desc) * p->count;
+  if (hipMalloc(&p->descs_dev, bytes) != hipSuccess) {
+    delete p;
+    return phpc_set_error(PHPC_ERR_MEM, "descriptor allocation failed");
+  }
+  // default choice: the planner's rule
+  int cfg = dtype == PHPC_F64 ? phpc_f64_pick_config(p->d64.data(), p->count) : 0;
+  if (dtype == PHPC_F32) {
+    long work = 0;
+    for (const auto &d : p->d32) work += (long)((d.m + 127) / 128) * ((d.n + 127) / 128);
+    cfg = work >= 2L * phpc_device().cus ? 2 : 0;
+  }
+  if (flags & PHPC_PLAN_AUTOTUNE) {
+    const std::string sig = signature(dtype, live.data(), (int)live.size());
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto hit = g_tune_cache.find(sig);
+    if (hit != g_tune_cache.end()) {
+      cfg = hit->second;
+    } else {
+      // save every C, time each candidate once (after one warm-up), restore C
+      const size_t esz = dtype == PHPC_F64 ? 8 : 4;
+      size_t cbytes = 0;
+      for (const auto &it : live) cbytes += (size_t)it.m * it.n * esz;
+      char *save = (char *)phpc_scratch(cbytes, 13);
+      if (!save) {
+        delete p;
+        return phpc_set_error(PHPC_ERR_MEM, "autotune scratch allocation failed");
+      }
+      size_t off = 0;
+      for (const auto &it : live) {
+        if (hipMemcpy2DAsync(save + off, it.n * esz, it.C, it.ldc * esz, it.n * esz, it.m, hipMemcpyDeviceToDevice,
+                             s) != hipSuccess) {
+          delete p;
+          return phpc_set_error(PHPC_ERR_HIP, "autotune save failed");
+        }
+        off += (size_t)it.m * it.n * esz;
+      }
+      const int ncfg = dtype == PHPC_F64 ? phpc_f64_num_configs() : phpc_f32_num_configs();
+      double best = 1e30;
+      int best_cfg = cfg;
+      hipEvent_t e0, e1;
+      (void)hipEventCreate(&e0);
+      (void)hipEventCreate(&e1);
+      for (int c = 0; c < ncfg; ++c) {
+        if (dtype == PHPC_F64 && mp && !phpc_f64_config_panels(c)) continue;
+        if (plan_upload(p, c, s) != PHPC_OK) continue;
+        p->grid = 0;
+        if (launch(p, s) != PHPC_OK) continue;
+        (void)hipEventRecord(e0, s);
+        if (launch(p, s) != PHPC_OK) continue;
+        (void)hipEventRecord(e1, s);
+        (void)hipEventSynchronize(e1);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best) {
+          best = ms;
+          best_cfg = c;
+        }
+      }
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      off = 0;
+      for (const auto &it : live) {
+        (void)hipMemcpy2DAsync(it.C, it.ldc * esz, save + off, it.n * esz, it.n * esz, it.m, hipMemcpyDeviceToDevice,
+                               s);
+        off += (size_t)it.m * it.n * esz;
+      }
+      (void)hipStreamSynchronize(s);
+      cfg = best_cfg;
+      p->tuned_ms = best;
+      g_tune_cache[sig] = cfg;
+    }
+  }
+  int rc = plan_upload(p, cfg, s);
+  if (rc) {
+    phpc_plan_destroy(p);
+    return rc;
+  }
+  p->grid = 0;
+  *out = p;
+  return phpc_set_error(PHPC_OK, nullptr);
+}
+
+extern "C" int phpc_plan_launch(phpc_plan_t p, void *stream) {
+  if (!p) return phpc_set_error(PHPC_ERR_ARG, "null plan");
+  int rc = launch(p, (hipStream_t)stream);
+  if (rc) return rc;
+  return phpc_set_error(PHPC_OK, nullptr);
+}
+
+extern "C" int phpc_plan_graph_launch(phpc_plan_t p, void *stream) {
+  if (!p || !stream) return phpc_set_error(PHPC_ERR_ARG, "graph launch needs a plan and a created stream");
+  hipStream_t s = (hipStream_t)stream;
+  if (!p->exec) {
+    PHPC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = launch(p, s);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(s, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (ec != hipSuccess) return phpc_set_error(PHPC_ERR_HIP, "stream capture failed: %s", hipGetErrorString(ec));
+    p->graph = g;
+    PHPC_HIP(hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0));
+    p->graph_stream = s;
+  }
+  PHPC_HIP(hipGraphLaunch(p->exec, s));
+  return phpc_set_error(PHPC_OK, nullptr);
+}
+
+extern "C" int phpc_plan_info(phpc_plan_t p, int *config, int *tiles, int *grid, double *tuned_ms) {
+  if (!p) return phpc_set_error(PHPC_ERR_ARG, "null plan");
+  if (config) *config = p->config;
+  if (tiles) *tiles = p->total_tiles;
+  if (grid) *grid = p->grid;
+  if (tuned_ms) *tuned_ms = p->tuned_ms;
+  return PHPC_OK;
+}
+
+extern "C" void phpc_plan_destroy(phpc_plan_t p) {
+  if (!p) return;
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  if (p->descs_dev) {
+    (void)hipDeviceSynchronize();
+    (void)hipFree(p->descs_dev);
+  }
+  delete p;
+}

@@ -38,6 +38,27 @@ struct GemmDesc {
   int tile_start;  // first global tile id of this GEMM (prefix sum)
 };
 
+// One FP32 GEMM of a grouped launch (row-major, C += A*B).
+struct F32Desc {
+  const float *A;
+  const float *B;
+  float *C;
+  long lda, ldb, ldc;
+  int m, n, k;
+  int tiles_n;
+  int tile_start;
+};
+int phpc_f32_num_configs();
+void phpc_f32_config_tile(int id, int *bm, int *bn);
+int phpc_f32_plan_tiles(F32Desc *descs, int ndesc, int id);
+bool phpc_f32_vec4_ok(const F32Desc *descs, int ndesc);
+int phpc_f32_launch(const F32Desc *descs_dev, const F32Desc *descs_host, int ndesc, int total_tiles, int id, bool vec4,
+                    int grid, hipStream_t stream);
+
+// FP64: fill tiles_n / tile_start for config id; returns the total tile count.
+int phpc_f64_plan_tiles(GemmDesc *descs, int ndesc, int id);
+bool phpc_f64_config_panels(int id);
+
 // Kernel-level tile configuration chosen by the planner / autotuner.
 struct F64Config {
   int id;         // index into the instantiated kernel table

@@ -53,6 +53,12 @@ class SplitInfo(ctypes.Structure):
                 ("error_estimate", ctypes.c_double), ("method", ctypes.c_int), ("int_bits", ctypes.c_int)]
 
 
+class GemmItem(ctypes.Structure):
+    _fields_ = [("A", ctypes.c_void_p), ("B", ctypes.c_void_p), ("C", ctypes.c_void_p),
+                ("lda", ctypes.c_long), ("ldb", ctypes.c_long), ("ldc", ctypes.c_long),
+                ("m", ctypes.c_int), ("n", ctypes.c_int), ("k", ctypes.c_int), ("panel", ctypes.c_int)]
+
+
 GEMM_T = ctypes.CFUNCTYPE(None, c_double_p, I, c_double_p, I, c_double_p, I, I, I, I, I, I, I, I, c_float_p)
 
 _SIGS = {
@@ -92,6 +98,12 @@ _SIGS = {
     "phpc_split_set_config": (I, [I]),
     "phpc_split_num_configs": (I, []),
     "phpc_split_set_method": (I, [I]),
+    # include/phpc_grouped.h
+    "phpc_plan_create": (I, [I, ctypes.POINTER(GemmItem), I, I, vp, ctypes.POINTER(vp)]),
+    "phpc_plan_launch": (I, [vp, vp]),
+    "phpc_plan_graph_launch": (I, [vp, vp]),
+    "phpc_plan_info": (I, [vp, c_int_p, c_int_p, c_int_p, c_double_p]),
+    "phpc_plan_destroy": (None, [vp]),
     # include/utils.h
     "get_cur_time": (ctypes.c_double, []),
 }
@@ -293,3 +305,48 @@ def dgemm_split_device(A, B, C, slices=0, target=1e-14, stream=None, method=None
     check(lib().phpc_dgemm_split_device(dptr(A), A.stride(0), dptr(B), B.stride(0), dptr(C), C.stride(0),
                                         m, k, n, slices, target, stream_ptr(stream), ctypes.byref(info)))
     return info
+
+
+F64 = 0
+F32 = 1
+PLAN_AUTOTUNE = 1
+
+
+class Plan:
+    """Owning wrapper of phpc_plan_t over torch CUDA tensors (A, B, C[, panel]) tuples."""
+
+    def __init__(self, dtype, gemms, autotune=False, stream=None):
+        items = (GemmItem * max(1, len(gemms)))()
+        self._keep = []
+        for i, g in enumerate(gemms):
+            A, B, C = g[0], g[1], g[2]
+            panel = g[3] if len(g) > 3 else 0
+            items[i] = GemmItem(A.data_ptr(), B.data_ptr(), C.data_ptr(), A.stride(0), B.stride(0), C.stride(0),
+                                A.shape[0], B.shape[1], A.shape[1], panel)
+            self._keep.append((A, B, C))
+        h = vp()
+        check(lib().phpc_plan_create(dtype, items, len(gemms), PLAN_AUTOTUNE if autotune else 0,
+                                     stream_ptr(stream), ctypes.byref(h)))
+        self.h = h
+
+    def launch(self, stream=None):
+        check(lib().phpc_plan_launch(self.h, stream_ptr(stream)))
+
+    def graph_launch(self, stream):
+        check(lib().phpc_plan_graph_launch(self.h, stream_ptr(stream)))
+
+    def info(self):
+        c, t, g, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        check(lib().phpc_plan_info(self.h, ctypes.byref(c), ctypes.byref(t), ctypes.byref(g), ctypes.byref(ms)))
+        return {"config": c.value, "tiles": t.value, "grid": g.value, "tuned_ms": ms.value}
+
+    def close(self):
+        if self.h:
+            lib().phpc_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
