@@ -79,6 +79,116 @@ def traffic_from_profiles(workload):
         return None
 
 
+def poisson_run(srv, submit, n_req, rate, load, acc, flops, iso, rng):
+    """Submit n_req requests at Poisson arrival times (rate per second) from this thread;
+    a second thread waits for them in order."""
+    import threading
+
+    import numpy as np
+
+    arr = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    tickets = [None] * n_req
+    got = [None] * n_req
+    ready = threading.Semaphore(0)
+
+    def waiter():
+        for i in range(n_req):
+            ready.acquire()
+            got[i] = srv.wait(tickets[i])
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    t0 = time.perf_counter()
+    for i in range(n_req):
+        dt = arr[i] - (time.perf_counter() - t0)
+        if dt > 0:
+            time.sleep(dt)
+        tickets[i] = submit(i)
+        ready.release()
+    th.join()
+    span = time.perf_counter() - t0
+    lat = np.array([g["latency_s"] for g in got])
+    slow = lat / iso
+    cls = {}
+    for a, tag in ((0, "fp64"), (1, "fp32")):
+        sel = acc == a
+        cls[tag] = {"p50_ms": round(float(np.percentile(lat[sel], 50)) * 1e3, 3),
+                    "p99_ms": round(float(np.percentile(lat[sel], 99)) * 1e3, 3)}
+    return {
+        "load": load, "rate_req_per_s": round(rate, 2),
+        "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+        "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+        "max_ms": round(float(lat.max()) * 1e3, 3),
+        "p99_vs_isolated_p99": round(float(np.percentile(lat, 99) / np.percentile(iso, 99)), 3),
+        "slowdown_p50": round(float(np.percentile(slow, 50)), 3),
+        "slowdown_p99": round(float(np.percentile(slow, 99)), 3),
+        "tflops": round(flops.sum() / span / 1e12, 3),
+        "mean_batch": round(float(np.mean([g["batch_size"] for g in got])), 2),
+        "by_class": cls,
+    }
+
+
+def serving_bench(phpc, n_req, seed=11, slots=2, batch_max=32, split_min=4096, loads=(0.3, 0.7, 0.9)):
+    """GEMM serving stream: square sizes 60% from [256,1024], 30% from [1024,4096], 10% from
+    [4096,8192]; accuracy class 50% FP64-level, 50% FP32-level; operands generated on the
+    device from per-request seeds; admission under a 200 GB budget. Measures each
+    request alone (isolated latency), the saturated rate (all requests at once), then
+    Poisson arrivals at `loads` x the saturated rate under each scheduling policy."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    u = rng.random(n_req)
+    size = np.where(u < 0.6, rng.integers(256, 1025, n_req),
+                    np.where(u < 0.9, rng.integers(1024, 4097, n_req), rng.integers(4096, 8193, n_req)))
+    acc = np.where(rng.random(n_req) < 0.5, phpc.ACC_FP64, phpc.ACC_FP32)
+    flops = 2.0 * size.astype(np.float64) ** 3
+    reqs = [(int(acc[i]), int(size[i]), 1000 + 2 * i) for i in range(n_req)]
+    srv = phpc.Server(200 * 10 ** 9, slots=slots, max_dim=8192, batch_max=batch_max)
+    srv.set_split_min(split_min)
+
+    def submit(i):
+        a, n, sd = reqs[i]
+        return srv.submit(a, n, n, n, phpc.DIST_UNIFORM, sd)
+
+    try:
+        for i in range(min(8, n_req)):  # first-use set-up of every kernel family
+            srv.wait(submit(i))
+        iso = np.empty(n_req)
+        for i in range(n_req):
+            iso[i] = srv.wait(submit(i))["latency_s"]
+        t0 = time.perf_counter()
+        ts = [submit(i) for i in range(n_req)]
+        res = [srv.wait(t) for t in ts]
+        t_sat = time.perf_counter() - t0
+        sat_rate = n_req / t_sat
+        _, peak = srv.memory()
+        out = {
+            "requests": n_req, "slots": slots, "batch_max": batch_max, "budget_bytes": 200 * 10 ** 9,
+            "fp64_split_min_dim": split_min,
+            "mix": "sizes 60% [256,1024], 30% [1024,4096], 10% [4096,8192]; 50% FP64-level, 50% FP32-level",
+            "isolated_ms": {"p50": round(float(np.percentile(iso, 50)) * 1e3, 3),
+                            "p99": round(float(np.percentile(iso, 99)) * 1e3, 3),
+                            "sum_s": round(float(iso.sum()), 3)},
+            "saturated": {"req_per_s": round(sat_rate, 2), "tflops": round(flops.sum() / t_sat / 1e12, 3),
+                          "mean_batch": round(float(np.mean([r["batch_size"] for r in res])), 2),
+                          "peak_operand_bytes": peak},
+            "poisson": {},
+        }
+        for policy, aging, tag in ((phpc.SCHED_FIFO, 0.0, "fifo"), (phpc.SCHED_SJF, 0.0, "sjf"),
+                                   (phpc.SCHED_SJF, 8.0, "sjf_aging8")):
+            srv.set_policy(policy, aging)
+            out["poisson"][tag] = []
+            for load in loads:
+                out["poisson"][tag].append(poisson_run(srv, submit, n_req, load * sat_rate, load, acc, flops, iso,
+                                                       np.random.default_rng(seed + int(load * 100))))
+                r = out["poisson"][tag][-1]
+                log(f"serving {tag} load {load}: p50 {r['p50_ms']} ms, p99 {r['p99_ms']} ms, "
+                    f"p99/iso p99 {r['p99_vs_isolated_p99']}")
+        return out
+    finally:
+        srv.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +199,8 @@ def main():
     ap.add_argument("--panel", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-vendor", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy three times (counter calibration)")
     args = ap.parse_args()
@@ -223,6 +335,8 @@ def main():
 
     del A, B, C
     torch.cuda.empty_cache()
+    if not args.no_serve:
+        extra["serving"] = serving_bench(phpc, args.serve_requests)
 
     workload = f"summa_fp64_n{N}_grid{p}x{q}_panel{panel}"
     out = {

@@ -17,6 +17,7 @@
 namespace {
 
 typedef float f16v __attribute__((ext_vector_type(16)));
+typedef float f4v __attribute__((ext_vector_type(4)));
 typedef __amdgpu_buffer_rsrc_t rsrc_t;
 typedef float __attribute__((address_space(1))) gfloat;
 
@@ -120,9 +121,11 @@ __global__ __launch_bounds__(256) void f32_gemm_kernel(F32Batch batch) {
         const int row = q / (BK / VEC), kc = (q % (BK / VEC)) * VEC;
         const uint32_t off = (row < m_rem && kc < krem) ? (uint32_t)(((long)row * lda + kb + kc) * 4) : kOOB;
         if constexpr (VEC == 4) {
-          const auto v = __builtin_amdgcn_raw_buffer_load_b128(rA, off, 0, 0);
+          // bit-cast the whole 128-bit result (element-wise extraction from the builtin's
+          // integer vector is lowered to a single-dword load)
+          const f4v v = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(rA, off, 0, 0));
 #pragma unroll
-          for (int e = 0; e < 4; ++e) ra[i][e] = __builtin_bit_cast(float, v[e]);
+          for (int e = 0; e < 4; ++e) ra[i][e] = v[e];
         } else {
           ra[i][0] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rA, off, 0, 0));
         }
@@ -134,9 +137,9 @@ __global__ __launch_bounds__(256) void f32_gemm_kernel(F32Batch batch) {
         const int kr = q / (BN / VEC), cc = (q % (BN / VEC)) * VEC;
         const uint32_t off = (kr < krem && cc < n_rem) ? (uint32_t)(((long)kr * ldb + cc) * 4) : kOOB;
         if constexpr (VEC == 4) {
-          const auto v = __builtin_amdgcn_raw_buffer_load_b128(rB, off, 0, 0);
+          const f4v v = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(rB, off, 0, 0));
 #pragma unroll
-          for (int e = 0; e < 4; ++e) rb[i][e] = __builtin_bit_cast(float, v[e]);
+          for (int e = 0; e < 4; ++e) rb[i][e] = v[e];
         } else {
           rb[i][0] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rB, off, 0, 0));
         }

@@ -662,6 +662,8 @@ GemmDesc *g_desc_dev = nullptr;
 size_t g_desc_cap = 0;
 GemmDesc *g_desc_pinned = nullptr;
 size_t g_pinned_cap = 0;
+std::mutex g_desc_mu;  // guards the descriptor buffers below (never held with g_mu)
+hipEvent_t g_desc_done = nullptr;  // recorded after the last launch that read g_desc_dev
 
 // Note: config 4's multi-panel VEC1 fallback uses the 8-wave kernel, whose launch
 // shape differs; launch() below takes the thread count from the kernel actually used.
@@ -796,30 +798,35 @@ int phpc_f64_grouped(const GemmDesc *descs_in, int ndesc, int grid, int config_i
     live.push_back(d);
   }
   if (live.empty()) return PHPC_OK;
-  const GemmDesc *dev = nullptr;
-  if (live.size() > 1) {
-    std::lock_guard<std::mutex> lk(g_mu);
-    const size_t bytes = live.size() * sizeof(GemmDesc);
-    if (bytes > g_desc_cap) {
-      if (g_desc_dev) {
-        PHPC_HIP(hipStreamSynchronize(stream));
-        PHPC_HIP(hipFree(g_desc_dev));
-      }
-      g_desc_cap = std::max(bytes, (size_t)64 * sizeof(GemmDesc));
-      PHPC_HIP(hipMalloc(&g_desc_dev, g_desc_cap));
-    }
-    if (bytes > g_pinned_cap) {
-      if (g_desc_pinned) PHPC_HIP(hipHostFree(g_desc_pinned));
-      g_pinned_cap = std::max(bytes, (size_t)64 * sizeof(GemmDesc));
-      PHPC_HIP(hipHostMalloc(&g_desc_pinned, g_pinned_cap, hipHostMallocDefault));
-    }
-    // The pinned staging copy may still be read by an earlier async copy.
-    PHPC_HIP(hipStreamSynchronize(stream));
-    memcpy(g_desc_pinned, live.data(), bytes);
-    PHPC_HIP(hipMemcpyAsync(g_desc_dev, g_desc_pinned, bytes, hipMemcpyHostToDevice, stream));
-    dev = g_desc_dev;
+  if (live.size() == 1) return phpc_f64_grouped_dev(nullptr, live.data(), 1, grid, config_id, stream);
+  // Several GEMMs: descriptors go through one process-wide device buffer. Callers on
+  // other streams may still have a launch queued that reads it, so the buffer is
+  // rewritten only after the previous user's launch has finished (g_desc_done).
+  std::lock_guard<std::mutex> lk(g_desc_mu);
+  const size_t bytes = live.size() * sizeof(GemmDesc);
+  if (g_desc_done) PHPC_HIP(hipEventSynchronize(g_desc_done));
+  if (bytes > g_desc_cap) {
+    if (g_desc_dev) PHPC_HIP(hipFree(g_desc_dev));
+    g_desc_dev = nullptr;
+    g_desc_cap = 0;
+    const size_t cap = std::max(bytes, (size_t)64 * sizeof(GemmDesc));
+    PHPC_HIP(hipMalloc(&g_desc_dev, cap));
+    g_desc_cap = cap;
   }
-  return phpc_f64_grouped_dev(dev, live.data(), (int)live.size(), grid, config_id, stream);
+  if (bytes > g_pinned_cap) {
+    if (g_desc_pinned) PHPC_HIP(hipHostFree(g_desc_pinned));
+    g_desc_pinned = nullptr;
+    g_pinned_cap = 0;
+    const size_t cap = std::max(bytes, (size_t)64 * sizeof(GemmDesc));
+    PHPC_HIP(hipHostMalloc(&g_desc_pinned, cap, hipHostMallocDefault));
+    g_pinned_cap = cap;
+  }
+  if (!g_desc_done) PHPC_HIP(hipEventCreateWithFlags(&g_desc_done, hipEventDisableTiming));
+  memcpy(g_desc_pinned, live.data(), bytes);
+  PHPC_HIP(hipMemcpyAsync(g_desc_dev, g_desc_pinned, bytes, hipMemcpyHostToDevice, stream));
+  const int rc = phpc_f64_grouped_dev(g_desc_dev, live.data(), (int)live.size(), grid, config_id, stream);
+  PHPC_HIP(hipEventRecord(g_desc_done, stream));
+  return rc;
 }
 
 // Tuning knobs for the autotuner: 0 = config for large panel-grouped problems, 1 =

@@ -102,3 +102,22 @@ int phpc_pack_launch(const double *src, long lds, double *dst, long ldd, int row
 
 // Scratch device buffer owned by the library (grows; stream-ordered use only).
 void *phpc_scratch(size_t bytes, int slot);
+
+// A set of growable device buffers for one caller at a time (the split path needs four).
+// Growing synchronises `stream` before the old buffer is freed.
+struct Workspace {
+  static constexpr int kSlots = 8;
+  void *p[kSlots] = {};
+  size_t cap[kSlots] = {};
+  void *get(size_t bytes, int slot, hipStream_t stream);
+  size_t total() const;
+  void release();
+  ~Workspace() { release(); }
+};
+
+#include "phpc_split.h"
+// Split-precision GEMM with an explicit workspace and method (the public entry point
+// uses a process-wide workspace and the method set by phpc_split_set_method).
+int phpc_dgemm_split_ws(const double *dA, long lda, const double *dB, long ldb, double *dC, long ldc, int m, int k,
+                        int n, int slices, double target_err, int method, hipStream_t stream, Workspace *ws,
+                        phpc_split_info *info);

@@ -102,6 +102,40 @@ void *phpc_scratch(size_t bytes, int slot) {
   return s.p;
 }
 
+void *Workspace::get(size_t bytes, int slot, hipStream_t stream) {
+  if (slot < 0 || slot >= kSlots) return nullptr;
+  if (bytes == 0) bytes = 1;
+  if (bytes > cap[slot]) {
+    if (p[slot]) {
+      (void)hipStreamSynchronize(stream);
+      (void)hipFree(p[slot]);
+      p[slot] = nullptr;
+      cap[slot] = 0;
+    }
+    if (hipMalloc(&p[slot], bytes) != hipSuccess) {
+      p[slot] = nullptr;
+      return nullptr;
+    }
+    cap[slot] = bytes;
+  }
+  return p[slot];
+}
+
+size_t Workspace::total() const {
+  size_t t = 0;
+  for (int i = 0; i < kSlots; ++i) t += cap[i];
+  return t;
+}
+
+void Workspace::release() {
+  for (int i = 0; i < kSlots; ++i)
+    if (p[i]) {
+      (void)hipFree(p[i]);
+      p[i] = nullptr;
+      cap[i] = 0;
+    }
+}
+
 extern "C" int phpc_set_launch_policy(int honor_grid, int config_id, int grid) {
   if (config_id >= phpc_f64_num_configs()) return phpc_set_error(PHPC_ERR_ARG, "config id %d out of range", config_id);
   g_policy.honor_grid = honor_grid ? 1 : 0;
@@ -296,6 +330,40 @@ extern "C" int phpc_fill_device(double *dX, long ld, int m, int n, int dist, uin
   long blocks = (total + 255) / 256;
   if (blocks > 65536) blocks = 65536;
   hipLaunchKernelGGL(fill_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, dX, ld, m, n, dist, seed);
+  PHPC_HIP(hipGetLastError());
+  return phpc_set_error(PHPC_OK, nullptr);
+}
+
+__global__ void fill_f32_kernel(float *x, long ld, int m, int n, int dist, uint64_t seed) {
+  const long total = (long)m * n;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    const long r = e / n, c = e % n;
+    double v;
+    if (dist == 2) {
+      v = (double)e;
+    } else {
+      const uint64_t z = mix64(seed + (uint64_t)(e + 1) * 0x9E3779B97F4A7C15ULL);
+      if (dist == 0) {
+        v = 2.0 * ((double)(z >> 11) * 0x1.0p-53) - 1.0;
+      } else {
+        const int ex = (int)(((z >> 32) * 41ULL) >> 32) - 20;
+        v = ldexp((z & 1) ? -1.0 : 1.0, ex);
+      }
+    }
+    x[r * ld + c] = (float)v;
+  }
+}
+
+extern "C" int phpc_fill_device_f32(float *dX, long ld, int m, int n, int dist, uint64_t seed, void *stream) {
+  if (phpc_init()) return phpc_last_error();
+  if (m < 0 || n < 0 || ld < n || (m > 0 && n > 0 && !dX) || dist < 0 || dist > 2)
+    return phpc_set_error(PHPC_ERR_ARG, "bad fill arguments");
+  if ((long)m * n == 0) return phpc_set_error(PHPC_OK, nullptr);
+  const long total = (long)m * n;
+  long blocks = (total + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(fill_f32_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, dX, ld, m, n, dist,
+                     seed);
   PHPC_HIP(hipGetLastError());
   return phpc_set_error(PHPC_OK, nullptr);
 }

@@ -800,17 +800,33 @@ extern "C" double phpc_split_error_estimate(int slices, double ratio_a, double r
   return split_error_estimate(slices, ratio_a, ratio_b, k);
 }
 
+namespace {
+std::mutex g_default_ws_mu;
+Workspace g_default_ws;
+}  // namespace
+
 extern "C" int phpc_dgemm_split_device(const double *dA, long lda, const double *dB, long ldb, double *dC, long ldc,
                                        int m, int k, int n, int slices, double target_err, void *stream_,
                                        phpc_split_info *info) {
+  // the process-wide workspace serves one call at a time
+  std::lock_guard<std::mutex> lk(g_default_ws_mu);
+  return phpc_dgemm_split_ws(dA, lda, dB, ldb, dC, ldc, m, k, n, slices, target_err,
+                             method_crt ? PHPC_SPLIT_CRT : PHPC_SPLIT_SLICES, (hipStream_t)stream_, &g_default_ws,
+                             info);
+}
+
+int phpc_dgemm_split_ws(const double *dA, long lda, const double *dB, long ldb, double *dC, long ldc, int m, int k,
+                        int n, int slices, double target_err, int method, hipStream_t stream_, Workspace *ws,
+                        phpc_split_info *info) {
+  const bool use_crt = method == PHPC_SPLIT_CRT;
   if (info) *info = phpc_split_info{};
   if (phpc_init()) return phpc_last_error();
   if (m < 0 || k < 0 || n < 0) return phpc_set_error(PHPC_ERR_ARG, "negative size");
   if ((m > 0 && k > 0 && (!dA || lda < k)) || (k > 0 && n > 0 && (!dB || ldb < n)) || (m > 0 && n > 0 && (!dC || ldc < n)))
     return phpc_set_error(PHPC_ERR_ARG, "bad pointers or leading dimensions");
-  if (!method_crt && (slices < 0 || slices > kMaxSlices))
+  if (!use_crt && (slices < 0 || slices > kMaxSlices))
     return phpc_set_error(PHPC_ERR_ARG, "slices must be 0 (auto) or 1..%d", kMaxSlices);
-  if (method_crt && (slices < 0 || slices == 1 || slices > kMaxMods))
+  if (use_crt && (slices < 0 || slices == 1 || slices > kMaxMods))
     return phpc_set_error(PHPC_ERR_ARG, "moduli must be 0 (auto) or 2..%d", kMaxMods);
   if (m == 0 || n == 0 || k == 0) return phpc_set_error(PHPC_OK, nullptr);
   hipStream_t s = (hipStream_t)stream_;
@@ -818,7 +834,7 @@ extern "C" int phpc_dgemm_split_device(const double *dA, long lda, const double 
 
   // scratch: exponents, ratios, column stats (slices and the FP64 accumulator below)
   const size_t small = (size_t)(m + n) * (sizeof(int) + sizeof(double)) + (size_t)n * 16 + 64;
-  char *meta = (char *)phpc_scratch(small, 11);
+  char *meta = (char *)ws->get(small, 3, s);
   if (!meta) return phpc_set_error(PHPC_ERR_MEM, "split scratch allocation failed");
   int *eA = (int *)meta;
   int *eB = eA + m;
@@ -843,7 +859,7 @@ extern "C" int phpc_dgemm_split_device(const double *dA, long lda, const double 
   hipLaunchKernelGGL(col_exps_kernel, dim3((n + 255) / 256), dim3(256), 0, s, cmax, csum, k, n, eB, ratioB);
   PHPC_HIP(hipGetLastError());
 
-  if (method_crt) {
+  if (use_crt) {
     // CRT scheme: choose the integer width p, then the modulus count
     std::vector<double> ra(m), rb(n);
     int hbad = 0;
@@ -871,9 +887,9 @@ extern "C" int phpc_dgemm_split_device(const double *dA, long lda, const double 
     if (pbits < 1) return phpc_set_error(PHPC_ERR_ARG, "k = %d too large for the CRT scheme", k);
     const CrtConsts cc = make_crt(L);
     const long ld8 = kp * L;
-    int8_t *A8 = (int8_t *)phpc_scratch((size_t)m * ld8, 8);
-    int8_t *B8 = (int8_t *)phpc_scratch((size_t)n * ld8, 9);
-    int8_t *R8 = (int8_t *)phpc_scratch((size_t)L * m * n, 12);
+    int8_t *A8 = (int8_t *)ws->get((size_t)m * ld8, 0, s);
+    int8_t *B8 = (int8_t *)ws->get((size_t)n * ld8, 1, s);
+    int8_t *R8 = (int8_t *)ws->get((size_t)L * m * n, 4, s);
     if (!A8 || !B8 || !R8) return phpc_set_error(PHPC_ERR_MEM, "CRT scratch allocation failed");
     {
       int rc = crt_dispatch_t<2>(L, s, dA, lda, dB, ldb, m, k, n, pbits, kp, eA, eB, cc, A8, B8, ld8, 0, nullptr,
@@ -945,9 +961,9 @@ extern "C" int phpc_dgemm_split_device(const double *dA, long lda, const double 
     est = split_error_estimate(s_use, mra, mrb, k);
   }
   const long ld8a = kp * s_use, ld8b = kp * s_use;
-  int8_t *A8 = (int8_t *)phpc_scratch((size_t)m * ld8a, 8);
-  int8_t *B8 = (int8_t *)phpc_scratch((size_t)n * ld8b, 9);
-  double *acc = (double *)phpc_scratch((size_t)m * n * 8, 10);
+  int8_t *A8 = (int8_t *)ws->get((size_t)m * ld8a, 0, s);
+  int8_t *B8 = (int8_t *)ws->get((size_t)n * ld8b, 1, s);
+  double *acc = (double *)ws->get((size_t)m * n * 8, 2, s);
   if (!A8 || !B8 || !acc) return phpc_set_error(PHPC_ERR_MEM, "split scratch allocation failed");
   hipLaunchKernelGGL(slice_rows_kernel, dim3(m), dim3(256), 0, s, dA, lda, k, s_use, kp, eA, A8, ld8a);
   PHPC_HIP(hipGetLastError());

@@ -59,6 +59,17 @@ class GemmItem(ctypes.Structure):
                 ("m", ctypes.c_int), ("n", ctypes.c_int), ("k", ctypes.c_int), ("panel", ctypes.c_int)]
 
 
+class Request(ctypes.Structure):
+    _fields_ = [("accuracy", ctypes.c_int), ("m", ctypes.c_int), ("n", ctypes.c_int), ("k", ctypes.c_int),
+                ("dist", ctypes.c_int), ("seed", ctypes.c_uint64), ("result_host", ctypes.c_void_p)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("batch_size", ctypes.c_int), ("latency_s", ctypes.c_double),
+                ("queue_s", ctypes.c_double), ("compute_s", ctypes.c_double), ("error_estimate", ctypes.c_double),
+                ("fill_s", ctypes.c_double), ("gemm_s", ctypes.c_double)]
+
+
 GEMM_T = ctypes.CFUNCTYPE(None, c_double_p, I, c_double_p, I, c_double_p, I, I, I, I, I, I, I, I, c_float_p)
 
 _SIGS = {
@@ -75,6 +86,7 @@ _SIGS = {
     "phpc_set_tuning": (I, [I, I]),
     "phpc_config_info": (I, [I, c_int_p, c_int_p, c_int_p]),
     "phpc_fill_device": (I, [vp, L, I, I, I, ctypes.c_uint64, vp]),
+    "phpc_fill_device_f32": (I, [vp, L, I, I, I, ctypes.c_uint64, vp]),
     "phpc_check_sampled": (I, [vp, L, vp, L, vp, L, vp, L, I, I, I, c_int_p, c_int_p, I,
                                c_double_p, c_double_p, vp]),
     # include/phpc_summa.h
@@ -104,6 +116,15 @@ _SIGS = {
     "phpc_plan_graph_launch": (I, [vp, vp]),
     "phpc_plan_info": (I, [vp, c_int_p, c_int_p, c_int_p, c_double_p]),
     "phpc_plan_destroy": (None, [vp]),
+    # include/phpc_serve.h
+    "phpc_server_create": (I, [ctypes.c_size_t, I, I, I, ctypes.POINTER(vp)]),
+    "phpc_server_submit": (I, [vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]),
+    "phpc_server_wait": (I, [vp, ctypes.c_uint64, ctypes.POINTER(Result)]),
+    "phpc_server_drain": (I, [vp]),
+    "phpc_server_set_split_min": (I, [vp, I]),
+    "phpc_server_set_policy": (I, [vp, I, ctypes.c_double]),
+    "phpc_server_memory": (I, [vp, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
+    "phpc_server_destroy": (None, [vp]),
     # include/utils.h
     "get_cur_time": (ctypes.c_double, []),
 }
@@ -244,8 +265,11 @@ def stream_ptr(stream=None):
 
 
 def fill_device(x, dist, seed, stream=None):
+    """Fill a float64 or float32 device matrix with a generated stream (phpc_fill_device)."""
+    import torch
     m, n = x.shape
-    check(lib().phpc_fill_device(dptr(x), x.stride(0), m, n, dist, seed, stream_ptr(stream)))
+    fn = lib().phpc_fill_device_f32 if x.dtype == torch.float32 else lib().phpc_fill_device
+    check(fn(dptr(x), x.stride(0), m, n, dist, seed, stream_ptr(stream)))
 
 
 def dgemm_device(A, B, C, panel=0, stream=None, m=None, k=None, n=None):
@@ -343,6 +367,64 @@ class Plan:
     def close(self):
         if self.h:
             lib().phpc_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------- serving loop (include/phpc_serve.h) ----------------
+
+ACC_FP64 = 0
+ACC_FP32 = 1
+SCHED_FIFO = 0
+SCHED_SJF = 1
+
+
+class Server:
+    """Owning wrapper of phpc_server_t. submit() returns a ticket; wait(ticket) blocks
+    (without holding the GIL) and returns the result as a dict."""
+
+    def __init__(self, budget_bytes, slots=2, max_dim=4096, batch_max=16):
+        h = vp()
+        check(lib().phpc_server_create(int(budget_bytes), slots, max_dim, batch_max, ctypes.byref(h)))
+        self.h = h
+
+    def submit(self, accuracy, m, n, k, dist=DIST_UNIFORM, seed=1, out=None):
+        """out: optional host numpy array (float64 for ACC_FP64, float32 for ACC_FP32, m x n)."""
+        req = Request(accuracy, m, n, k, dist, seed, out.ctypes.data if out is not None else None)
+        t = ctypes.c_uint64()
+        check(lib().phpc_server_submit(self.h, ctypes.byref(req), ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket):
+        r = Result()
+        rc = lib().phpc_server_wait(self.h, ticket, ctypes.byref(r))
+        check(rc)
+        return {"status": r.status, "batch_size": r.batch_size, "latency_s": r.latency_s, "queue_s": r.queue_s,
+                "compute_s": r.compute_s, "error_estimate": r.error_estimate, "fill_s": r.fill_s,
+                "gemm_s": r.gemm_s}
+
+    def drain(self):
+        check(lib().phpc_server_drain(self.h))
+
+    def set_split_min(self, dim):
+        check(lib().phpc_server_set_split_min(self.h, dim))
+
+    def set_policy(self, policy, aging=0.0):
+        check(lib().phpc_server_set_policy(self.h, policy, aging))
+
+    def memory(self):
+        a, b = ctypes.c_size_t(), ctypes.c_size_t()
+        check(lib().phpc_server_memory(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def close(self):
+        if self.h:
+            lib().phpc_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

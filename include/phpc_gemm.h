@@ -100,6 +100,8 @@ int phpc_config_info(int id, int *bm, int *bn, int *threads);
  * Fills an m x n block with leading dimension ld, element (r, c) using index r*n + c.
  */
 int phpc_fill_device(double *dX, long ld, int m, int n, int dist, uint64_t seed, void *stream);
+/* The same values rounded to FP32. */
+int phpc_fill_device_f32(float *dX, long ld, int m, int n, int dist, uint64_t seed, void *stream);
 
 /*
  * Sampled accuracy check on the device, for any C += A*B product computed on the GPU:

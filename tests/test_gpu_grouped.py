@@ -49,6 +49,26 @@ def test_f32_mixed_shapes_one_launch(autotune):
     assert info["tiles"] > 0 and (info["tuned_ms"] > 0) == autotune
 
 
+@pytest.mark.parametrize("shapes", [[(512, 768, 256)], [(64, 64, 64)], [(128, 128, 16)],
+                                    [(2048, 2048, 256)], [(512, 768, 256), (64, 64, 64), (4, 8, 12)]])
+def test_f32_vec4_loads(shapes):
+    """Every dimension a multiple of 4 and 16-byte aligned rows: the 128-bit load path."""
+    import torch
+    gemms, host = [], []
+    for i, (m, n, k) in enumerate(shapes):
+        A = O.uniform((m, k), 40 + i).astype(np.float32)
+        B = O.uniform((k, n), 50 + i).astype(np.float32)
+        C0 = O.uniform((m, n), 60 + i).astype(np.float32)
+        gemms.append((torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(), torch.from_numpy(C0.copy()).cuda()))
+        host.append((A, B, C0))
+    plan = phpc.Plan(phpc.F32, gemms)
+    plan.launch()
+    torch.cuda.synchronize()
+    for (_, _, dC), (A, B, C0) in zip(gemms, host):
+        ref = C0.astype(np.float64) + A.astype(np.float64) @ B.astype(np.float64)
+        assert np.all(np.abs(dC.cpu().numpy().astype(np.float64) - ref) <= f32_bound(A, B, C0))
+
+
 def test_f64_grouped_items_bitexact():
     import torch
     shapes = [(130, 70, 90), (1, 1, 1), (257, 3, 33), (64, 64, 64), (300, 129, 17)]
