@@ -79,6 +79,157 @@ def traffic_from_profiles(workload):
         return None
 
 
+def grouped_bench(phpc, seed=21):
+    """(a) N = 10000 FP64 on an emulated 3 x 3 grid (rank blocks 3334/3333/3333), panel
+    N / lcm = 3333, and (b) 512 independent FP32 GEMMs with M, N, K log-uniform in
+    [64, 4096], each set in one persistent launch replayed from a captured hipGraph, with
+    the tile configuration chosen by the autotuner."""
+    import numpy as np
+    import torch
+
+    out = {}
+    s = torch.cuda.Stream()
+    # (a)
+    N, p, q = 10000, 3, 3
+    A = torch.empty((N, N), dtype=torch.float64, device="cuda")
+    B = torch.empty_like(A)
+    phpc.fill_device(A, phpc.DIST_UNIFORM, seed)
+    phpc.fill_device(B, phpc.DIST_UNIFORM, seed + 1)
+    C = torch.zeros_like(A)
+    starts = [0, 3334, 6667, 10000]
+    items = [(A[starts[i]:starts[i + 1]], B[:, starts[j]:starts[j + 1]],
+              C[starts[i]:starts[i + 1], starts[j]:starts[j + 1]], N // 3) for i in range(p) for j in range(q)]
+    plan = phpc.Plan(phpc.F64, items, autotune=True, stream=s)
+    C.zero_()
+    torch.cuda.synchronize()
+    with torch.cuda.stream(s):
+        plan.graph_launch(s)  # capture + first replay
+    s.synchronize()
+    C.zero_()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 3
+    with torch.cuda.stream(s):
+        e0.record(s)
+        for _ in range(reps):
+            plan.graph_launch(s)
+        e1.record(s)
+    s.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    C.zero_()
+    with torch.cuda.stream(s):
+        plan.graph_launch(s)
+    s.synchronize()
+    rng = np.random.default_rng(seed)
+    rows = np.concatenate([rng.integers(0, N, 2048), [3333, 3334, 6666, 6667, 9999]])
+    cols = np.concatenate([rng.integers(0, N, 2048), [3333, 3334, 6666, 6667, 9999]])
+    cw, nw = phpc.check_sampled(A, B, C, rows, cols)
+    info = plan.info()
+    out["fp64_n10000_grid3x3"] = {"tflops": round(2.0 * N ** 3 / (ms * 1e-3) / 1e12, 3), "ms": round(ms, 3),
+                                  "blocks": "3334/3333/3333", "panel": N // 3, "config": info["config"],
+                                  "tiles": info["tiles"], "err_cw_max_sampled": cw, "err_nw_sampled": nw,
+                                  "launch": "one persistent kernel per pass, hipGraph replay"}
+    plan.close()
+    del A, B, C, items
+    torch.cuda.empty_cache()
+    # (b)
+    rng = np.random.default_rng(seed)
+    shapes = np.rint(np.exp(rng.uniform(np.log(64), np.log(4096), size=(512, 3)))).astype(int)
+    gemms = []
+    flops = 0.0
+    for i, (m, n, k) in enumerate(shapes):
+        a = torch.empty((m, k), dtype=torch.float32, device="cuda")
+        b = torch.empty((k, n), dtype=torch.float32, device="cuda")
+        phpc.fill_device(a, phpc.DIST_UNIFORM, 10000 + 2 * i)
+        phpc.fill_device(b, phpc.DIST_UNIFORM, 10001 + 2 * i)
+        gemms.append((a, b, torch.zeros((m, n), dtype=torch.float32, device="cuda")))
+        flops += 2.0 * m * n * k
+    plan = phpc.Plan(phpc.F32, gemms, autotune=True, stream=s)
+    for g in gemms:
+        g[2].zero_()
+    torch.cuda.synchronize()
+    with torch.cuda.stream(s):
+        plan.graph_launch(s)
+    s.synchronize()
+    with torch.cuda.stream(s):
+        e0.record(s)
+        for _ in range(reps):
+            plan.graph_launch(s)
+        e1.record(s)
+    s.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    for g in gemms:
+        g[2].zero_()
+    with torch.cuda.stream(s):
+        plan.graph_launch(s)
+    s.synchronize()
+    # accuracy: 16 GEMMs x 64 entries against float64 dot products of the FP32 inputs
+    worst = 0.0
+    for gi in rng.choice(512, 16, replace=False):
+        a, b, c = gemms[gi]
+        m, k = a.shape
+        n = b.shape[1]
+        r = rng.integers(0, m, 64)
+        cc = rng.integers(0, n, 64)
+        ar = a[r].double().cpu().numpy()
+        bc = b[:, cc].double().cpu().numpy()
+        ref = np.einsum("ik,ki->i", ar, bc)
+        scale = np.einsum("ik,ki->i", np.abs(ar), np.abs(bc))
+        got = c[r, cc].double().cpu().numpy()
+        worst = max(worst, float(np.max(np.abs(got - ref) / np.maximum(scale, 1e-300))))
+    info = plan.info()
+    out["fp32_512_gemms"] = {"tflops": round(flops / (ms * 1e-3) / 1e12, 3), "ms": round(ms, 3),
+                             "flops": flops, "config": info["config"], "tiles": info["tiles"],
+                             "tuned_ms": round(info["tuned_ms"], 3), "err_cw_max_sampled": worst,
+                             "fp32_mfma_util": round(flops / (ms * 1e-3) / 157.3e12, 4),
+                             "launch": "one persistent kernel for all 512 GEMMs, hipGraph replay"}
+    plan.close()
+    del gemms
+    torch.cuda.empty_cache()
+    return out
+
+
+def split_sweep(phpc, sizes=(4096, 8192), seed=31):
+    """Split-precision FP64 GEMM at each slice / modulus count: error against the
+    double-double product on sampled entries and FP64-equivalent TFLOP/s."""
+    import numpy as np
+    import torch
+
+    out = []
+    for N in sizes:
+        A = torch.empty((N, N), dtype=torch.float64, device="cuda")
+        B = torch.empty_like(A)
+        C = torch.zeros_like(A)
+        rng = np.random.default_rng(seed)
+        rows, cols = rng.integers(0, N, 2048), rng.integers(0, N, 2048)
+        for dist, tag in ((phpc.DIST_UNIFORM, "uniform"), (phpc.DIST_WIDE, "wide_exponent")):
+            phpc.fill_device(A, dist, seed)
+            phpc.fill_device(B, dist, seed + 1)
+            runs = [(phpc.SPLIT_CRT, 0)] + [(phpc.SPLIT_CRT, L) for L in (8, 10, 12, 14, 16)] + \
+                   [(phpc.SPLIT_SLICES, s_) for s_ in (3, 4, 5, 6, 7, 8)]
+            for method, count in runs:
+                C.zero_()
+                info = phpc.dgemm_split_device(A, B, C, slices=count, target=1e-14, method=method)
+                torch.cuda.synchronize()
+                cw, nw = phpc.check_sampled(A, B, C, rows, cols)
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for _ in range(2):
+                    phpc.dgemm_split_device(A, B, C, slices=count, target=1e-14, method=method)
+                e1.record()
+                torch.cuda.synchronize()
+                ms = e0.elapsed_time(e1) / 2
+                out.append({"N": N, "inputs": tag, "method": "crt" if method == phpc.SPLIT_CRT else "slices",
+                            "requested": count if count else "auto", "count": info.slices,
+                            "int8_gemms": info.products, "int_bits": info.int_bits, "err_cw_max_sampled": cw,
+                            "error_estimate": info.error_estimate,
+                            "tflops_fp64_equiv": round(2.0 * N ** 3 / (ms * 1e-3) / 1e12, 3), "ms": round(ms, 3)})
+        del A, B, C
+        torch.cuda.empty_cache()
+    phpc.lib().phpc_split_set_method(phpc.SPLIT_CRT)
+    return out
+
+
 def poisson_run(srv, submit, n_req, rate, load, acc, flops, iso, rng):
     """Submit n_req requests at Poisson arrival times (rate per second) from this thread;
     a second thread waits for them in order."""
@@ -128,7 +279,7 @@ def poisson_run(srv, submit, n_req, rate, load, acc, flops, iso, rng):
     }
 
 
-def serving_bench(phpc, n_req, seed=11, slots=2, batch_max=32, split_min=4096, loads=(0.3, 0.7, 0.9)):
+def serving_bench(phpc, n_req, seed=11, slots=3, batch_max=32, split_min=4096, loads=(0.3, 0.7, 0.9)):
     """GEMM serving stream: square sizes 60% from [256,1024], 30% from [1024,4096], 10% from
     [4096,8192]; accuracy class 50% FP64-level, 50% FP32-level; operands generated on the
     device from per-request seeds; admission under a 200 GB budget. Measures each
@@ -200,6 +351,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-vendor", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-grouped", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy three times (counter calibration)")
@@ -335,6 +488,10 @@ def main():
 
     del A, B, C
     torch.cuda.empty_cache()
+    if not args.no_grouped:
+        extra["grouped"] = grouped_bench(phpc)
+    if not args.no_sweep:
+        extra["split_sweep"] = split_sweep(phpc)
     if not args.no_serve:
         extra["serving"] = serving_bench(phpc, args.serve_requests)
 

@@ -390,6 +390,15 @@ int phpc_pack_launch(const double *src, long lds, double *dst, long ldd, int row
   return PHPC_OK;
 }
 
+extern "C" int phpc_pack_device(const double *src, long ld_src, double *dst, long ld_dst, int rows, int cols,
+                                void *stream) {
+  if (phpc_init()) return phpc_last_error();
+  if (rows < 0 || cols < 0 || ld_src < cols || ld_dst < cols || (rows > 0 && cols > 0 && (!src || !dst)))
+    return phpc_set_error(PHPC_ERR_ARG, "bad block descriptor");
+  int rc = phpc_pack_launch(src, ld_src, dst, ld_dst, rows, cols, (hipStream_t)stream);
+  return rc ? rc : phpc_set_error(PHPC_OK, nullptr);
+}
+
 // ---------------------------------------------------------------------------
 // sampled double-double accuracy check (one 64-lane workgroup per sampled entry)
 // ---------------------------------------------------------------------------
@@ -481,4 +490,47 @@ extern "C" int phpc_check_sampled(const double *dA, long lda, const double *dB, 
   if (err_cw) *err_cw = cw;
   if (err_nw) *err_nw = mr > 0 ? md / mr : md;
   return phpc_set_error(PHPC_OK, nullptr);
+}
+
+extern "C" int phpc_check_sampled_host(const double *A, long lda, const double *B, long ldb, const double *C, long ldc,
+                                       int m, int k, int n, int count, uint64_t seed, double *err_cw, double *err_nw) {
+  if (phpc_init()) return phpc_last_error();
+  if (!A || !B || !C || m < 1 || n < 1 || k < 1 || lda < k || ldb < n || ldc < n || count < 1)
+    return phpc_set_error(PHPC_ERR_ARG, "bad check arguments");
+  // sampled entries from the seed (SplitMix64 stream), plus the four corners
+  std::vector<int> rows, cols;
+  const int corners[4][2] = {{0, 0}, {0, n - 1}, {m - 1, 0}, {m - 1, n - 1}};
+  for (auto &c : corners) {
+    rows.push_back(c[0]);
+    cols.push_back(c[1]);
+  }
+  for (int e = 0; e < count; ++e) {
+    uint64_t z = seed + (uint64_t)(e + 1) * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    rows.push_back((int)((z >> 32) % (uint64_t)m));
+    cols.push_back((int)((z & 0xffffffffu) % (uint64_t)n));
+  }
+  double *dA = nullptr, *dB = nullptr, *dC = nullptr;
+  int rc = PHPC_OK;
+  if (hipMalloc(&dA, sizeof(double) * (size_t)m * k) != hipSuccess ||
+      hipMalloc(&dB, sizeof(double) * (size_t)k * n) != hipSuccess ||
+      hipMalloc(&dC, sizeof(double) * (size_t)m * n) != hipSuccess) {
+    rc = phpc_set_error(PHPC_ERR_MEM, "check buffers");
+  }
+  if (!rc && (hipMemcpy2D(dA, sizeof(double) * k, A, sizeof(double) * lda, sizeof(double) * k, m,
+                          hipMemcpyHostToDevice) != hipSuccess ||
+              hipMemcpy2D(dB, sizeof(double) * n, B, sizeof(double) * ldb, sizeof(double) * n, k,
+                          hipMemcpyHostToDevice) != hipSuccess ||
+              hipMemcpy2D(dC, sizeof(double) * n, C, sizeof(double) * ldc, sizeof(double) * n, m,
+                          hipMemcpyHostToDevice) != hipSuccess))
+    rc = phpc_set_error(PHPC_ERR_HIP, "check copies");
+  if (!rc)
+    rc = phpc_check_sampled(dA, k, dB, n, nullptr, 0, dC, n, m, k, n, rows.data(), cols.data(), (int)rows.size(),
+                            err_cw, err_nw, nullptr);
+  if (dA) (void)hipFree(dA);
+  if (dB) (void)hipFree(dB);
+  if (dC) (void)hipFree(dC);
+  return rc;
 }

@@ -131,6 +131,7 @@ struct phpc_server {
   std::unordered_map<uint64_t, Req *> all;
   uint64_t next_id = 1;
   int pending = 0;
+  int big_running = 0;  // requests that fill the GPU alone, in flight (at most one)
   bool stop = false;
   std::vector<Worker *> workers;
 
@@ -141,17 +142,21 @@ struct phpc_server {
   void process(Worker *w, std::vector<Req *> &batch);
 };
 
-// The next request to run: the oldest (FIFO), or the one with the least estimated
-// device time, less its waiting time divided by the aging factor (shortest first).
+// The next request to run, or queue.end() if none may start now: the oldest (FIFO), or
+// the one with the least estimated device time less its waiting time divided by the
+// aging factor (shortest first). A request that fills the GPU alone does not start
+// while another one runs (two of them would only share the GPU and both finish late);
+// small requests may start beside it.
 std::deque<Req *>::iterator phpc_server::pick_locked() {
-  if (policy == PHPC_SCHED_FIFO) return queue.begin();
   const clk::time_point now = clk::now();
-  auto best = queue.begin();
+  auto best = queue.end();
   double best_key = 0.0;
   for (auto it = queue.begin(); it != queue.end(); ++it) {
+    if (big_running > 0 && !(*it)->small) continue;
+    if (policy == PHPC_SCHED_FIFO) return it;
     double key = (*it)->est;
     if (aging > 0.0) key -= std::chrono::duration<double>(now - (*it)->arrive).count() / aging;
-    if (it == queue.begin() || key < best_key) {
+    if (best == queue.end() || key < best_key) {
       best = it;
       best_key = key;
     }
@@ -284,13 +289,12 @@ void phpc_server::run(Worker *w) {
       std::unique_lock<std::mutex> lk(mu);
       Req *h = nullptr;
       for (;;) {
-        if (!queue.empty()) {
-          auto it = pick_locked();
-          if (admit_locked(*it)) {
-            h = *it;
-            queue.erase(it);
-            break;
-          }
+        auto it = pick_locked();
+        if (it != queue.end() && admit_locked(*it)) {
+          h = *it;
+          queue.erase(it);
+          if (!h->small) ++big_running;
+          break;
         }
         if (stop) return;  // nothing admissible left for this worker
         cv_work.wait(lk);
@@ -315,6 +319,7 @@ void phpc_server::run(Worker *w) {
     process(w, batch);
     {
       std::lock_guard<std::mutex> lk(mu);
+      if (!batch[0]->small) --big_running;
       for (Req *q : batch) {
         for (int i = 0; i < 3; ++i) in_use -= round_up(q->bytes[i]);
         release_locked(q);

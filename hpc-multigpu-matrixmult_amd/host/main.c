@@ -13,7 +13,11 @@
  *   --panel W        k-panel width (default N / lcm(p, q), as the reference)
  *   --dist D         ramp (default, A[i] = B[i] = i as src/main.c:85-86), uniform, wide
  *   --seed S         generator seed for uniform / wide (A uses S, B uses S + 1)
- *   --mode M         fused (default) or stepwise
+ *   --mode M         fused (default), stepwise, chained or split (FP64-accurate
+ *                    split-precision GEMM on int8 matrix cores)
+ *   --check S        after the HIP pass, check S sampled entries of C (plus the
+ *                    corners) against a double-double dot product; exit status 3
+ *                    if any exceeds the FP64 dot-product bound (N + 2) * 2^-53
  *   --uneven         allow N not divisible by the grid (the reference aborts, :47-52)
  *   --no-vendor      skip the rocBLAS pass (cublas_time column = 0)
  *
@@ -64,7 +68,7 @@ static void fill(double *x, size_t n, const char *dist, uint64_t seed) {
 int main(int argc, char *argv[]) {
   int pos = 0;
   const char *posv[5];
-  int nranks = 1, gp = 0, gq = 0, panel = 0, uneven = 0, vendor = 1, mode = PHPC_SUMMA_FUSED;
+  int nranks = 1, gp = 0, gq = 0, panel = 0, uneven = 0, vendor = 1, mode = PHPC_SUMMA_FUSED, check = 0;
   const char *dist = "ramp";
   uint64_t seed = 1;
   for (int i = 1; i < argc; ++i) {
@@ -91,7 +95,21 @@ int main(int argc, char *argv[]) {
       seed = strtoull(v, NULL, 10);
       ++i;
     } else if (!strcmp(o, "--mode") && v) {
-      mode = !strcmp(v, "stepwise") ? PHPC_SUMMA_STEPWISE : PHPC_SUMMA_FUSED;
+      if (!strcmp(v, "fused")) {
+        mode = PHPC_SUMMA_FUSED;
+      } else if (!strcmp(v, "stepwise")) {
+        mode = PHPC_SUMMA_STEPWISE;
+      } else if (!strcmp(v, "chained")) {
+        mode = PHPC_SUMMA_CHAINED;
+      } else if (!strcmp(v, "split")) {
+        mode = PHPC_SUMMA_SPLIT;
+      } else {
+        fprintf(stderr, "Error: --mode must be fused, stepwise, chained or split\n");
+        return EXIT_FAILURE;
+      }
+      ++i;
+    } else if (!strcmp(o, "--check") && v) {
+      check = atoi(v);
       ++i;
     } else if (!strcmp(o, "--uneven")) {
       uneven = 1;
@@ -173,6 +191,20 @@ int main(int argc, char *argv[]) {
     return EXIT_FAILURE;
   }
 
+  int check_failed = 0;
+  if (check > 0) {
+    double err_cw = 0.0, err_nw = 0.0;
+    rc = phpc_check_sampled_host(A, N, B, N, C, N, N, N, N, check, seed + 7, &err_cw, &err_nw);
+    if (rc != PHPC_OK) {
+      fprintf(stderr, "Error: %s\n", phpc_last_error_message());
+      return EXIT_FAILURE;
+    }
+    const double bound = (N + 2.0) * 0x1.0p-53;
+    check_failed = !(err_cw <= bound);
+    fprintf(stderr, "check: %d samples, max componentwise error %.3e, normwise %.3e (bound %.3e) %s\n", check + 4,
+            err_cw, err_nw, bound, check_failed ? "FAILED" : "ok");
+  }
+
   /* vendor (cuBLAS in the reference, rocBLAS here) pass */
   double cublas_time = 0.0;
   if (vendor) {
@@ -195,5 +227,5 @@ int main(int argc, char *argv[]) {
   free(B);
   free(C);
   phpc_grid_destroy(grid);
-  return 0;
+  return check_failed ? 3 : 0;
 }

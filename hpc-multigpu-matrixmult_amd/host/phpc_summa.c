@@ -20,7 +20,7 @@ int phpc_set_error(int code, const char *fmt, ...);
 struct phpc_grid {
   int p, q;
   int panel; /* 0: N / lcm(p, q) */
-  int mode;  /* PHPC_SUMMA_FUSED or PHPC_SUMMA_STEPWISE */
+  int mode;  /* PHPC_SUMMA_FUSED, _STEPWISE, _CHAINED or _SPLIT */
 };
 
 /* src/phpc_summa.c:9-22 */
@@ -81,7 +81,8 @@ int phpc_grid_set_panel(phpc_grid_t grid, int panel) {
 int phpc_grid_get_panel(phpc_grid_t grid) { return grid ? grid->panel : -1; }
 
 int phpc_grid_set_mode(phpc_grid_t grid, int mode) {
-  if (!grid || (mode != PHPC_SUMMA_FUSED && mode != PHPC_SUMMA_STEPWISE && mode != PHPC_SUMMA_CHAINED))
+  if (!grid || (mode != PHPC_SUMMA_FUSED && mode != PHPC_SUMMA_STEPWISE && mode != PHPC_SUMMA_CHAINED &&
+                mode != PHPC_SUMMA_SPLIT))
     return PHPC_ERR_ARG;
   grid->mode = mode;
   return PHPC_OK;

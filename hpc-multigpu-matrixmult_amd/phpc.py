@@ -34,6 +34,7 @@ SUMMA_FUSED = 0
 SUMMA_STEPWISE = 1
 SUMMA_VENDOR = 2
 SUMMA_CHAINED = 3
+SUMMA_SPLIT = 4
 
 DIST_UNIFORM = 0
 DIST_WIDE = 1
@@ -89,6 +90,7 @@ _SIGS = {
     "phpc_fill_device_f32": (I, [vp, L, I, I, I, ctypes.c_uint64, vp]),
     "phpc_check_sampled": (I, [vp, L, vp, L, vp, L, vp, L, I, I, I, c_int_p, c_int_p, I,
                                c_double_p, c_double_p, vp]),
+    "phpc_check_sampled_host": (I, [vp, L, vp, L, vp, L, I, I, I, I, ctypes.c_uint64, c_double_p, c_double_p]),
     # include/phpc_summa.h
     "phpc_grid_create": (I, [I, ctypes.POINTER(vp)]),
     "phpc_grid_create_dims": (I, [I, I, ctypes.POINTER(vp)]),
@@ -103,6 +105,7 @@ _SIGS = {
     "phpc_gemm_summa": (I, [GEMM_T, vp, vp, vp, vp, I, I, I, I, I, c_float_p]),
     "phpc_summa_device": (I, [I, I, I, I, vp, vp, vp, I, I, vp]),
     "phpc_summa_host": (I, [I, I, I, I, vp, vp, vp, I, I, c_float_p]),
+    "phpc_pack_device": (I, [vp, L, vp, L, I, I, vp]),
     # include/phpc_split.h
     "phpc_dgemm_split_device": (I, [vp, L, vp, L, vp, L, I, I, I, I, ctypes.c_double, vp,
                                     ctypes.POINTER(SplitInfo)]),

@@ -114,6 +114,11 @@ int phpc_check_sampled(const double *dA, long lda, const double *dB, long ldb, c
                        long ldc0, const double *dC, long ldc, int m, int k, int n, const int *rows,
                        const int *cols, int count, double *err_cw, double *err_nw, void *stream);
 
+/* The same check of C = A * B on host matrices (copied to the device): `count` entries
+ * drawn from `seed` plus the four corners. Used by the CLI's --check. */
+int phpc_check_sampled_host(const double *A, long lda, const double *B, long ldb, const double *C, long ldc, int m,
+                            int k, int n, int count, uint64_t seed, double *err_cw, double *err_nw);
+
 #ifdef __cplusplus
 }
 #endif

@@ -40,12 +40,16 @@ int phpc_grid_get_panel(phpc_grid_t grid);
  *   PHPC_SUMMA_CHAINED  one persistent launch, one FMA chain per output over all panels
  *                       (C is not rounded between panels): the arithmetic of a single
  *                       local-GEMM call over the full k; the fastest FP64 schedule
+ *   PHPC_SUMMA_SPLIT    every rank's update over the full k by the split-precision CRT
+ *                       GEMM (include/phpc_split.h): FP64-accurate (err_cw <= 1e-14),
+ *                       int8 matrix cores
  * FUSED and STEPWISE produce bit-identical C (the reference's per-panel grouping). */
 #define PHPC_SUMMA_FUSED 0
 #define PHPC_SUMMA_STEPWISE 1
 #define PHPC_SUMMA_VENDOR 2
 #define PHPC_SUMMA_CHAINED 3
-int phpc_grid_set_mode(phpc_grid_t grid, int mode); /* FUSED, STEPWISE or CHAINED */
+#define PHPC_SUMMA_SPLIT 4
+int phpc_grid_set_mode(phpc_grid_t grid, int mode); /* FUSED, STEPWISE, CHAINED or SPLIT */
 
 /* Least common multiple, src/phpc_summa.c:9-22. */
 int find_lcm(int a, int b);
@@ -87,6 +91,11 @@ int phpc_gemm_summa(gemm_t f, phpc_grid_t grid, const double *A, const double *B
  * panel 0 = N / lcm(p, q). wg > 0 fixes the persistent grid. */
 int phpc_summa_device(int p, int q, int panel, int mode, const double *dA, const double *dB, double *dC, int N,
                       int wg, void *stream);
+
+/* Strided block -> packed block copy, the MPI_Type_vector panel and block types of
+ * src/phpc_summa.c:53-59 (count = rows, blocklength = cols, stride = ld_src) on device
+ * pointers: dst[r * ld_dst + c] = src[r * ld_src + c], bytes copied unchanged. */
+int phpc_pack_device(const double *src, long ld_src, double *dst, long ld_dst, int rows, int cols, void *stream);
 
 /* Host-pointer SUMMA through the device path: copies A, B, C in once, runs
  * phpc_summa_device, copies C out. *kernel_seconds = device time of the SUMMA. */

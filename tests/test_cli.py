@@ -38,3 +38,9 @@ def test_no_visible_device_fails_cleanly(tmp_path):
     assert r.returncode != 0
     assert "Error" in r.stderr
     assert not list((tmp_path / "csv").glob("*.csv"))
+
+
+def test_bad_mode(tmp_path):
+    r = run(["16", "32", "1", "1", "t", "--mode", "bogus"], tmp_path)
+    assert r.returncode != 0
+    assert "--mode must be" in r.stderr

@@ -149,3 +149,36 @@ def test_large_single_gemm_uses_wide_tile_and_stays_exact():
     got = dC[rows].cpu().numpy()
     want = O.kernel_emul(np.ascontiguousarray(A[rows]), B, np.zeros((len(rows), N)))
     assert np.array_equal(bits(got), bits(want))
+
+
+def test_pack_device_byte_exact():
+    """The strided -> packed block copy (MPI_Type_vector panel, src/phpc_summa.c:53-59)
+    moves bytes unchanged: NaN payloads, signed zeros, infinities, subnormals."""
+    import torch
+    rng = np.random.default_rng(9)
+    src = rng.integers(0, 2 ** 63, size=(70, 90), dtype=np.int64).view(np.float64).copy()
+    specials = np.array([np.nan, -0.0, np.inf, -np.inf, 5e-324, -5e-324], dtype=np.float64)
+    src.reshape(-1)[:6] = specials
+    src.view(np.uint64).reshape(-1)[6] = 0x7FF0000000000123  # signalling NaN payload
+    dsrc = torch.from_numpy(src).cuda()
+    for (r0, c0, rows, cols) in [(0, 0, 70, 90), (3, 5, 40, 33), (69, 89, 1, 1), (10, 0, 0, 5)]:
+        dst = torch.full((max(rows, 1), cols + 3), -1.0, dtype=torch.float64, device="cuda")
+        sub = dsrc[r0:, c0:]
+        phpc.check(phpc.lib().phpc_pack_device(phpc.dptr(sub), 90, phpc.dptr(dst), cols + 3, rows, cols,
+                                               phpc.stream_ptr()))
+        torch.cuda.synchronize()
+        got = dst.cpu().numpy().view(np.uint64)
+        assert np.array_equal(got[:rows, :cols], src[r0:r0 + rows, c0:c0 + cols].view(np.uint64))
+        assert np.all(dst.cpu().numpy()[:, cols:] == -1.0)  # padding columns untouched
+
+
+@pytest.mark.parametrize("p,q,N", [(2, 2, 256), (3, 3, 301), (1, 1, 200)])
+def test_summa_split_mode_fp64_accurate(p, q, N):
+    import torch
+    A, B = O.uniform((N, N), 31), O.uniform((N, N), 32)
+    dA, dB = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
+    dC = torch.zeros((N, N), dtype=torch.float64, device="cuda")
+    phpc.summa_device(dA, dB, dC, p, q, mode=phpc.SUMMA_SPLIT)
+    torch.cuda.synchronize()
+    cw, _ = O.err_full(dC.cpu().numpy(), O.gemm_dd(A, B), A, B)
+    assert cw <= 1e-14
