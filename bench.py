@@ -230,6 +230,52 @@ def split_sweep(phpc, sizes=(4096, 8192), seed=31):
     return out
 
 
+def fp32_accurate_bench(phpc, sizes=(4096, 8192, 16384), seed=41):
+    """FP32 inputs, uniform[-1,1): the FP32 matrix-core GEMM and the FP32-accurate 3 x BF16
+    GEMM (six BF16 slice products, FP32 accumulation). Error against float64 dot products
+    of the same FP32 inputs on sampled entries, normalised by |A||B|."""
+    import numpy as np
+    import torch
+
+    out = []
+    for N in sizes:
+        A = torch.empty((N, N), dtype=torch.float32, device="cuda")
+        B = torch.empty_like(A)
+        C = torch.zeros_like(A)
+        phpc.fill_device(A, phpc.DIST_UNIFORM, seed)
+        phpc.fill_device(B, phpc.DIST_UNIFORM, seed + 1)
+        rng = np.random.default_rng(seed)
+        r = rng.integers(0, N, 256)
+        c = rng.integers(0, N, 256)
+        ar = A[r].double().cpu().numpy()
+        bc = B[:, c].double().cpu().numpy()
+        ref = np.einsum("ik,ki->i", ar, bc)
+        scale = np.einsum("ik,ki->i", np.abs(ar), np.abs(bc))
+        row = {"N": N}
+        for method, tag in ((phpc.SGEMM_FP32, "fp32_mfma"), (phpc.SGEMM_BF16X3, "bf16x3")):
+            C.zero_()
+            phpc.sgemm_device(A, B, C, method=method)
+            torch.cuda.synchronize()
+            err = float(np.max(np.abs(C[r, c].double().cpu().numpy() - ref) / scale))
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            reps = 3
+            e0.record()
+            for _ in range(reps):
+                phpc.sgemm_device(A, B, C, method=method)
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            tf = 2.0 * N ** 3 / (ms * 1e-3) / 1e12
+            row[tag] = {"tflops": round(tf, 3), "ms": round(ms, 3), "err_cw_max_sampled": err,
+                        "mfma_util": round(tf * (6 if method == phpc.SGEMM_BF16X3 else 1) /
+                                           (2500.0 if method == phpc.SGEMM_BF16X3 else 157.3), 4)}
+        row["bf16x3_vs_fp32"] = round(row["bf16x3"]["tflops"] / row["fp32_mfma"]["tflops"], 3)
+        out.append(row)
+        del A, B, C
+        torch.cuda.empty_cache()
+    return out
+
+
 def poisson_run(srv, submit, n_req, rate, load, acc, flops, iso, rng):
     """Submit n_req requests at Poisson arrival times (rate per second) from this thread;
     a second thread waits for them in order."""
@@ -492,6 +538,7 @@ def main():
         extra["grouped"] = grouped_bench(phpc)
     if not args.no_sweep:
         extra["split_sweep"] = split_sweep(phpc)
+        extra["fp32_accurate"] = fp32_accurate_bench(phpc)
     if not args.no_serve:
         extra["serving"] = serving_bench(phpc, args.serve_requests)
 

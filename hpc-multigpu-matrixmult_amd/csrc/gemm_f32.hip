@@ -8,6 +8,7 @@
 // MFMA 32x32x2: lane l holds A[row l%32][k l/32] and B[k l/32][col l%32]; the 16
 // accumulators of lane l are rows 8(r/4) + 4(l/32) + r%4, column l%32.
 #include "phpc_internal.h"
+#include "phpc_split.h"
 
 #include <algorithm>
 #include <cstring>
@@ -210,6 +211,182 @@ __global__ __launch_bounds__(256) void f32_gemm_kernel(F32Batch batch) {
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// FP32-accurate GEMM from three BF16 slices (v_mfma_f32_32x32x16_bf16)
+// ---------------------------------------------------------------------------
+// Every FP32 operand is cut exactly into x = hi + mid + lo, each a BF16 value: hi is x
+// rounded to nearest BF16, mid the rounded remainder, lo the rest (|mid| <= 2^-8|x|,
+// |lo| <= 2^-16|x|; with round-to-nearest the remainder after two slices has at most 8
+// significant bits, so lo is exact for finite inputs whose slices stay normal). The six
+// products with slice indices summing to at most 2 (hi.hi, hi.mid, mid.hi, hi.lo,
+// mid.mid, lo.hi) run on the BF16 matrix cores with FP32 accumulation; the dropped
+// mid.lo, lo.mid and lo.lo terms are below 2^-23 |a||b|. The result carries the error
+// of an FP32 dot product: |C - exact| <= ((k + 2) 2^-24 + 2^-23) (|C0| + |A||B|).
+//
+// Operands are loaded as FP32 from HBM (raw buffer loads, zero outside the matrix),
+// split in registers and stored to LDS as three BF16 planes with k contiguous: A as
+// [plane][row][k], B transposed as [plane][col][k], so each MFMA operand is one 16-byte
+// LDS read (lane l: row / column l % 32, k = 8 (l / 32) .. + 7 of the 16-deep step).
+// The next slab's FP32 values are loaded into registers while the current one computes.
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+typedef unsigned short us8v __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ uint32_t bf16_rne(float x) {
+  const uint32_t u = __float_as_uint(x);
+  return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
+__device__ __forceinline__ float bf16_val(uint32_t h) { return __uint_as_float(h << 16); }
+
+template <int BM, int BN, int VEC>
+__global__ __launch_bounds__(256) void f32x3_gemm_kernel(F32Batch batch) {
+  constexpr int BK = 32;
+  constexpr int KP = BK + 8;  // bf16 per LDS row (80 bytes: 16-byte aligned, staggered banks)
+  constexpr int WTM = BM / 2, WTN = BN / 2;
+  constexpr int TM = WTM / 32, TN = WTN / 32;
+  constexpr int AE = BM * BK / 256;  // A floats per thread
+  constexpr int BE = BN * BK / 256;  // B floats per thread
+  static_assert(AE == 16 && BE == 16, "thread mapping assumes 128 x 128 tiles");
+  __shared__ __attribute__((aligned(16))) unsigned short lds[3 * BM * KP + 3 * BN * KP];
+  unsigned short *Al = lds;
+  unsigned short *Bl = lds + 3 * BM * KP;
+
+  const int tid = threadIdx.x;
+  const int wave = uni(tid >> 6);
+  const int lane = tid & 63;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int G = gridDim.x, bid = blockIdx.x;
+  const int slot = uni((G % 8) == 0 ? (bid % 8) * (G / 8) + bid / 8 : bid);
+  // A: thread -> row tid / 2, k offset (tid % 2) * 16; B: column tid % 128, k half tid / 128
+  const int a_row = tid >> 1, a_k = (tid & 1) * 16;
+  const int b_col = tid & 127, b_k = (tid >> 7) * 16;
+
+  for (int round = 0;; ++round) {
+    const int t = round * G + slot;
+    if (t >= batch.total_tiles) break;
+    const F32Desc d = uniform_desc(batch, uni(t));
+    const int lt = t - d.tile_start;
+    const int row0 = uni((lt / d.tiles_n) * BM);
+    const int col0 = uni((lt % d.tiles_n) * BN);
+    const int m_rem = d.m - row0, n_rem = d.n - col0;
+    const long lda = d.lda, ldb = d.ldb, ldc = d.ldc;
+    const float *Ag = d.A + (long)row0 * lda;
+    const float *Bg = d.B + col0;
+    gfloat *Cg = (gfloat *)(d.C + (long)row0 * ldc + col0);
+    const rsrc_t rA = make_rsrc(Ag, (long)min(m_rem, BM) * lda * 4);
+
+    f16v acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) acc[i][j] = f16v{};
+
+    float ra[16], rb[16];
+    auto load = [&](int kb) {
+      const int krem = d.k - kb;
+      if constexpr (VEC == 4) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int kc = a_k + 4 * q;
+          const uint32_t off = (a_row < m_rem && kc < krem) ? (uint32_t)(((long)a_row * lda + kb + kc) * 4) : kOOB;
+          const f4v v = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(rA, off, 0, 0));
+#pragma unroll
+          for (int e = 0; e < 4; ++e) ra[4 * q + e] = v[e];
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+          const int kc = a_k + e;
+          const uint32_t off = (a_row < m_rem && kc < krem) ? (uint32_t)(((long)a_row * lda + kb + kc) * 4) : kOOB;
+          ra[e] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rA, off, 0, 0));
+        }
+      }
+      const rsrc_t rB = make_rsrc(Bg + (long)kb * ldb, (long)krem * ldb * 4);
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int kr = b_k + e;
+        const uint32_t off = (kr < krem && b_col < n_rem) ? (uint32_t)(((long)kr * ldb + b_col) * 4) : kOOB;
+        rb[e] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rB, off, 0, 0));
+      }
+    };
+    // split 16 consecutive-k values into three 16-element BF16 runs at dst + plane * pstride
+    auto split_store = [&](const float *x, unsigned short *dst, int pstride) {
+      us8v h[2], m[2], l[2];
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const uint32_t hb = bf16_rne(x[e]);
+        const float r1 = x[e] - bf16_val(hb);
+        const uint32_t mb = bf16_rne(r1);
+        const float r2 = r1 - bf16_val(mb);
+        h[e >> 3][e & 7] = (unsigned short)hb;
+        m[e >> 3][e & 7] = (unsigned short)mb;
+        l[e >> 3][e & 7] = (unsigned short)bf16_rne(r2);
+      }
+#pragma unroll
+      for (int hlf = 0; hlf < 2; ++hlf) {
+        *(us8v *)(dst + 8 * hlf) = h[hlf];
+        *(us8v *)(dst + pstride + 8 * hlf) = m[hlf];
+        *(us8v *)(dst + 2 * pstride + 8 * hlf) = l[hlf];
+      }
+    };
+
+    const int nslab = (d.k + BK - 1) / BK;
+    load(0);
+    for (int sidx = 0; sidx < nslab; ++sidx) {
+      __syncthreads();  // the previous slab's MFMA reads are done
+      split_store(ra, Al + a_row * KP + a_k, BM * KP);
+      split_store(rb, Bl + b_col * KP + b_k, BN * KP);
+      __syncthreads();
+      if (sidx + 1 < nslab) load((sidx + 1) * BK);
+#pragma unroll
+      for (int kk = 0; kk < BK / 16; ++kk) {
+        const int ko = kk * 16 + 8 * (lane >> 5);
+        bf8v a[TM][3], b[TN][3];
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int p = 0; p < 3; ++p)
+            a[i][p] = __builtin_bit_cast(
+                bf8v, *(const us8v *)(Al + p * BM * KP + (wm * WTM + i * 32 + (lane & 31)) * KP + ko));
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+          for (int p = 0; p < 3; ++p)
+            b[j][p] = __builtin_bit_cast(
+                bf8v, *(const us8v *)(Bl + p * BN * KP + (wn * WTN + j * 32 + (lane & 31)) * KP + ko));
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) {
+            // small terms first
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i][2], b[j][0], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i][1], b[j][1], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i][0], b[j][2], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i][1], b[j][0], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i][0], b[j][1], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i][0], b[j][0], acc[i][j], 0, 0, 0);
+          }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const int cc = wn * WTN + j * 32 + (lane & 31);
+        if (cc >= n_rem) continue;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int rr = wm * WTM + i * 32 + 8 * (r / 4) + 4 * (lane >> 5) + (r % 4);
+          if (rr < m_rem) {
+            gfloat *pc = Cg + (long)rr * ldc + cc;
+            *pc = *pc + acc[i][j][r];
+          }
+        }
+      }
+    __syncthreads();  // LDS is rewritten by the next tile
+  }
+}
+
 typedef void (*F32Fn)(F32Batch);
 struct F32Cfg {
   int bm, bn;
@@ -274,6 +451,44 @@ bool phpc_f32_vec4_ok(const F32Desc *descs, int ndesc) {
   return true;
 }
 
+// FP32-accurate 3 x BF16 kernel: 128 x 128 tiles (plan with config 2's tile size).
+int phpc_f32x3_launch(const F32Desc *descs_dev, const F32Desc *descs_host, int ndesc, int total_tiles, bool vec4,
+                      int grid, hipStream_t stream) {
+  if (ndesc <= 0 || total_tiles <= 0) return PHPC_OK;
+  for (int i = 0; i < ndesc; ++i) {
+    const F32Desc &d = descs_host[i];
+    if (128L * d.lda * 4 >= 0x7fffffffL || 32L * d.ldb * 4 >= 0x7fffffffL)
+      return phpc_set_error(PHPC_ERR_ARG, "leading dimension too large");
+  }
+  int occ;
+  {
+    static int occ_cache[2] = {0, 0};
+    std::lock_guard<std::mutex> lk(g_mu);
+    const void *fn = vec4 ? reinterpret_cast<const void *>(f32x3_gemm_kernel<128, 128, 4>)
+                          : reinterpret_cast<const void *>(f32x3_gemm_kernel<128, 128, 1>);
+    if (occ_cache[vec4 ? 1 : 0] == 0) {
+      int n = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 256, 0) != hipSuccess || n < 1) n = 1;
+      occ_cache[vec4 ? 1 : 0] = n;
+    }
+    occ = occ_cache[vec4 ? 1 : 0];
+  }
+  F32Batch b{};
+  b.descs = descs_dev;
+  b.ndesc = ndesc;
+  b.total_tiles = total_tiles;
+  b.d0 = descs_host[0];
+  int G = grid > 0 ? grid : occ * phpc_device().cus;
+  if (G > total_tiles) G = total_tiles;
+  if (G < 1) G = 1;
+  if (vec4)
+    hipLaunchKernelGGL((f32x3_gemm_kernel<128, 128, 4>), dim3(G), dim3(256), 0, stream, b);
+  else
+    hipLaunchKernelGGL((f32x3_gemm_kernel<128, 128, 1>), dim3(G), dim3(256), 0, stream, b);
+  PHPC_HIP(hipGetLastError());
+  return PHPC_OK;
+}
+
 // Launch the grouped kernel over device-resident descriptors (planned for config id).
 int phpc_f32_launch(const F32Desc *descs_dev, const F32Desc *descs_host, int ndesc, int total_tiles, int id, bool vec4,
                     int grid, hipStream_t stream) {
@@ -294,4 +509,37 @@ int phpc_f32_launch(const F32Desc *descs_dev, const F32Desc *descs_host, int nde
   hipLaunchKernelGGL(kF32[id].fn[vec4 ? 1 : 0], dim3(G), dim3(256), 0, stream, b);
   PHPC_HIP(hipGetLastError());
   return PHPC_OK;
+}
+
+// Single FP32 GEMM on device pointers: FP32 matrix cores or the 3 x BF16 kernel.
+extern "C" int phpc_sgemm_device(const float *dA, long lda, const float *dB, long ldb, float *dC, long ldc, int m, int k,
+                                 int n, int method, void *stream) {
+  if (phpc_init()) return phpc_last_error();
+  if (method != PHPC_SGEMM_FP32 && method != PHPC_SGEMM_BF16X3) return phpc_set_error(PHPC_ERR_ARG, "bad method");
+  if (m < 0 || k < 0 || n < 0) return phpc_set_error(PHPC_ERR_ARG, "negative size");
+  if ((m > 0 && k > 0 && (!dA || lda < k)) || (k > 0 && n > 0 && (!dB || ldb < n)) || (m > 0 && n > 0 && (!dC || ldc < n)))
+    return phpc_set_error(PHPC_ERR_ARG, "bad pointers or leading dimensions");
+  if (m == 0 || n == 0 || k == 0) return phpc_set_error(PHPC_OK, nullptr);
+  F32Desc d{};
+  d.A = dA;
+  d.B = dB;
+  d.C = dC;
+  d.lda = lda;
+  d.ldb = ldb;
+  d.ldc = ldc;
+  d.m = m;
+  d.n = n;
+  d.k = k;
+  const bool vec4 = phpc_f32_vec4_ok(&d, 1);
+  int rc;
+  if (method == PHPC_SGEMM_BF16X3) {
+    const int tiles = phpc_f32_plan_tiles(&d, 1, 2);
+    rc = phpc_f32x3_launch(nullptr, &d, 1, tiles, vec4, 0, (hipStream_t)stream);
+  } else {
+    const long work = (long)((m + 127) / 128) * ((n + 127) / 128);
+    const int cfg = work >= 2L * phpc_device().cus ? 2 : 0;
+    const int tiles = phpc_f32_plan_tiles(&d, 1, cfg);
+    rc = phpc_f32_launch(nullptr, &d, 1, tiles, cfg, vec4, 0, (hipStream_t)stream);
+  }
+  return rc ? rc : phpc_set_error(PHPC_OK, nullptr);
 }

@@ -48,7 +48,8 @@ int plan_upload(phpc_plan *p, int id, hipStream_t s) {
     bytes = sizeof(GemmDesc) * p->count;
     src = p->d64.data();
   } else {
-    p->total_tiles = phpc_f32_plan_tiles(p->d32.data(), p->count, id);
+    // the 3 x BF16 kernel has one tile shape, 128 x 128 (config 2's)
+    p->total_tiles = phpc_f32_plan_tiles(p->d32.data(), p->count, p->dtype == PHPC_F32X3 ? 2 : id);
     p->vec = phpc_f32_vec4_ok(p->d32.data(), p->count);
     bytes = sizeof(F32Desc) * p->count;
     src = p->d32.data();
@@ -63,6 +64,9 @@ int launch(phpc_plan *p, hipStream_t s) {
   if (p->count == 0 || p->total_tiles == 0) return PHPC_OK;
   if (p->dtype == PHPC_F64)
     return phpc_f64_grouped_dev((const GemmDesc *)p->descs_dev, p->d64.data(), p->count, p->grid, p->config, s);
+  if (p->dtype == PHPC_F32X3)
+    return phpc_f32x3_launch((const F32Desc *)p->descs_dev, p->d32.data(), p->count, p->total_tiles, p->vec, p->grid,
+                             s);
   return phpc_f32_launch((const F32Desc *)p->descs_dev, p->d32.data(), p->count, p->total_tiles, p->config, p->vec,
                          p->grid, s);
 }
@@ -74,7 +78,7 @@ extern "C" int phpc_plan_create(int dtype, const phpc_gemm_item *items, int coun
   if (!out) return phpc_set_error(PHPC_ERR_ARG, "null plan pointer");
   *out = nullptr;
   if (phpc_init()) return phpc_last_error();
-  if ((dtype != PHPC_F64 && dtype != PHPC_F32) || count < 0 || (count > 0 && !items))
+  if ((dtype != PHPC_F64 && dtype != PHPC_F32 && dtype != PHPC_F32X3) || count < 0 || (count > 0 && !items))
     return phpc_set_error(PHPC_ERR_ARG, "bad plan arguments");
   hipStream_t s = (hipStream_t)stream_;
   phpc_plan *p = new phpc_plan();
@@ -84,7 +88,7 @@ extern "C" int phpc_plan_create(int dtype, const phpc_gemm_item *items, int coun
     const phpc_gemm_item &it = items[i];
     if (it.m < 0 || it.n < 0 || it.k < 0 || (it.m && it.k && (!it.A || it.lda < it.k)) ||
         (it.k && it.n && (!it.B || it.ldb < it.n)) || (it.m && it.n && (!it.C || it.ldc < it.n)) ||
-        (dtype == PHPC_F32 && it.panel > 0 && it.panel < it.k)) {
+        (dtype != PHPC_F64 && it.panel > 0 && it.panel < it.k)) {
       delete p;
       return phpc_set_error(PHPC_ERR_ARG, "bad GEMM item %d", i);
     }
@@ -126,7 +130,7 @@ extern "C" int phpc_plan_create(int dtype, const phpc_gemm_item *items, int coun
     *out = p;
     return phpc_set_error(PHPC_OK, nullptr);
   }
-  const size_t bytes = dtype == PHPC_F64 ? sizeof(GemmDesc) * p->count : sizeof(F32Desc) * p->count;
+  const size_t bytes = dtype == PHPC_F64 ? sizeof(GemmDesc) * p->count : sizeof(F32Desc) * p->count;  // F32, F32X3
   if (hipMalloc(&p->descs_dev, bytes) != hipSuccess) {
     delete p;
     return phpc_set_error(PHPC_ERR_MEM, "descriptor allocation failed");
@@ -138,6 +142,7 @@ extern "C" int phpc_plan_create(int dtype, const phpc_gemm_item *items, int coun
     for (const auto &d : p->d32) work += (long)((d.m + 127) / 128) * ((d.n + 127) / 128);
     cfg = work >= 2L * phpc_device().cus ? 2 : 0;
   }
+  if (dtype == PHPC_F32X3) cfg = 2;
   if (flags & PHPC_PLAN_AUTOTUNE) {
     const std::string sig = signature(dtype, live.data(), (int)live.size());
     std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -171,6 +176,7 @@ extern "C" int phpc_plan_create(int dtype, const phpc_gemm_item *items, int coun
       (void)hipEventCreate(&e1);
       for (int c = 0; c < ncfg; ++c) {
         if (dtype == PHPC_F64 && mp && !phpc_f64_config_panels(c)) continue;
+        if (dtype == PHPC_F32X3 && c != 2) continue;
         if (plan_upload(p, c, s) != PHPC_OK) continue;
         p->grid = 0;
         if (launch(p, s) != PHPC_OK) continue;

@@ -113,6 +113,7 @@ _SIGS = {
     "phpc_split_set_config": (I, [I]),
     "phpc_split_num_configs": (I, []),
     "phpc_split_set_method": (I, [I]),
+    "phpc_sgemm_device": (I, [vp, L, vp, L, vp, L, I, I, I, I, vp]),
     # include/phpc_grouped.h
     "phpc_plan_create": (I, [I, ctypes.POINTER(GemmItem), I, I, vp, ctypes.POINTER(vp)]),
     "phpc_plan_launch": (I, [vp, vp]),
@@ -139,6 +140,13 @@ def lib():
     if _lib is None:
         if not LIB_PATH.exists():
             raise RuntimeError(f"libphpc is not built: {LIB_PATH} missing (run __graft_entry__.build())")
+        # torch ships its own HIP runtime under the same soname. Loading torch first makes
+        # libphpc bind to that copy, so a process holds one HIP runtime; loading libphpc
+        # first would bring in the system copy and torch would then load a second one.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         h = ctypes.CDLL(str(LIB_PATH))
         for name, (res, args) in _SIGS.items():
             fn = getattr(h, name, None)
@@ -336,6 +344,17 @@ def dgemm_split_device(A, B, C, slices=0, target=1e-14, stream=None, method=None
 
 F64 = 0
 F32 = 1
+F32X3 = 2
+SGEMM_FP32 = 0
+SGEMM_BF16X3 = 1
+
+
+def sgemm_device(A, B, C, method=1, stream=None):
+    """FP32 C += A*B on torch float32 CUDA tensors (method SGEMM_FP32 or SGEMM_BF16X3)."""
+    m, k = A.shape
+    n = B.shape[1]
+    check(lib().phpc_sgemm_device(dptr(A), A.stride(0), dptr(B), B.stride(0), dptr(C), C.stride(0), m, k, n,
+                                  method, stream_ptr(stream)))
 PLAN_AUTOTUNE = 1
 
 

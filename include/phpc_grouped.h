@@ -18,6 +18,9 @@ extern "C" {
 
 #define PHPC_F64 0
 #define PHPC_F32 1
+/* FP32 operands, FP32-accurate result from three BF16 slices per operand on the BF16
+ * matrix cores (see phpc_sgemm_device) */
+#define PHPC_F32X3 2
 
 /* C[m x n] (ldc) += A[m x k] (lda) * B[k x n] (ldb) on device pointers of the plan's
  * element type. panel > 0 (FP64 only): group k in panels, as phpc_dgemm_device. */

@@ -62,6 +62,23 @@ double phpc_split_error_estimate(int slices, double ratio_a, double ratio_b, int
 int phpc_split_set_config(int cfg);
 int phpc_split_num_configs(void);
 
+/* ---------------- FP32-accurate GEMM from BF16 slices ----------------
+ * C[m x n] (ldc) += A[m x k] (lda) * B[k x n] (ldb), FP32 row-major device pointers,
+ * stream-ordered.
+ *   PHPC_SGEMM_FP32    FP32 matrix cores (v_mfma_f32_32x32x2f32)
+ *   PHPC_SGEMM_BF16X3  every operand cut exactly into three BF16 slices (hi + mid + lo,
+ *                      round to nearest); the six products hi.hi, hi.mid, mid.hi,
+ *                      hi.lo, mid.mid, lo.hi on the BF16 matrix cores
+ *                      (v_mfma_f32_32x32x16_bf16) with FP32 accumulation. Error:
+ *                      |C - exact| <= ((k + 2) 2^-24 + 2^-23) (|C0| + |A||B|), the FP32
+ *                      dot-product bound plus the dropped slice terms. Inputs must be
+ *                      finite with |x| < 3.39e38 (the largest BF16).
+ * No counterpart in the reference (FP64 only). */
+#define PHPC_SGEMM_FP32 0
+#define PHPC_SGEMM_BF16X3 1
+int phpc_sgemm_device(const float *dA, long lda, const float *dB, long ldb, float *dC, long ldc, int m, int k, int n,
+                      int method, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,97 @@
+"""FP32-accurate GEMM from three BF16 slices (and the FP32 matrix-core path) against a
+float64 reference of the same FP32 inputs, within the stated bound; exact on small
+integers; grouped plans replayed from a hipGraph equal eager launches."""
+import numpy as np
+import pytest
+
+from conftest import phpc
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def bound(A, B, C0):
+    k = A.shape[1]
+    mag = np.abs(C0).astype(np.float64) + np.abs(A.astype(np.float64)) @ np.abs(B.astype(np.float64))
+    return ((k + 2) * 2.0 ** -24 + 2.0 ** -23) * mag
+
+
+SHAPES = [(128, 128, 32), (256, 384, 512), (1, 1, 1), (130, 70, 33), (257, 129, 1000), (64, 512, 16),
+          (1000, 17, 999), (2048, 2048, 2048)]
+
+
+@pytest.mark.parametrize("method", [phpc.SGEMM_BF16X3, phpc.SGEMM_FP32])
+@pytest.mark.parametrize("m,n,k", SHAPES)
+def test_sgemm_within_fp32_bound(method, m, n, k):
+    import torch
+    A = O.uniform((m, k), 1).astype(np.float32)
+    B = O.uniform((k, n), 2).astype(np.float32)
+    C0 = O.uniform((m, n), 3).astype(np.float32)
+    dC = torch.from_numpy(C0.copy()).cuda()
+    phpc.sgemm_device(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(), dC, method=method)
+    torch.cuda.synchronize()
+    ref = C0.astype(np.float64) + A.astype(np.float64) @ B.astype(np.float64)
+    err = np.abs(dC.cpu().numpy().astype(np.float64) - ref)
+    assert np.all(err <= bound(A, B, C0)), (m, n, k, float(np.max(err / bound(A, B, C0))))
+
+
+def test_bf16x3_wide_range_inputs():
+    """Entries spread over 2^-40 .. 2^40: each slice keeps its own exponent."""
+    import torch
+    rng = np.random.default_rng(4)
+    m, n, k = 200, 160, 300
+    A = np.ldexp(rng.uniform(0.5, 1, (m, k)) * rng.choice([-1, 1], (m, k)), rng.integers(-40, 40, (m, k)))
+    B = np.ldexp(rng.uniform(0.5, 1, (k, n)) * rng.choice([-1, 1], (k, n)), rng.integers(-40, 40, (k, n)))
+    A, B = A.astype(np.float32), B.astype(np.float32)
+    C0 = np.zeros((m, n), np.float32)
+    dC = torch.zeros((m, n), dtype=torch.float32, device="cuda")
+    phpc.sgemm_device(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(), dC, method=phpc.SGEMM_BF16X3)
+    torch.cuda.synchronize()
+    ref = A.astype(np.float64) @ B.astype(np.float64)
+    assert np.all(np.abs(dC.cpu().numpy() - ref) <= bound(A, B, C0))
+
+
+def test_bf16x3_exact_on_small_integers():
+    """Integers |x| <= 8 are single BF16 slices; products and sums below 2^24 are exact."""
+    import torch
+    rng = np.random.default_rng(5)
+    m, n, k = 300, 200, 512
+    A = rng.integers(-8, 9, (m, k)).astype(np.float32)
+    B = rng.integers(-8, 9, (k, n)).astype(np.float32)
+    dC = torch.zeros((m, n), dtype=torch.float32, device="cuda")
+    phpc.sgemm_device(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(), dC, method=phpc.SGEMM_BF16X3)
+    torch.cuda.synchronize()
+    want = (A.astype(np.int64) @ B.astype(np.int64)).astype(np.float32)
+    assert np.array_equal(dC.cpu().numpy(), want)
+
+
+def test_bf16x3_plan_graph_replays_equal_eager():
+    import torch
+    shapes = [(300, 200, 150), (512, 768, 256), (64, 64, 64), (1000, 17, 999)]
+    gem_g, gem_e, host = [], [], []
+    for i, (m, n, k) in enumerate(shapes):
+        A = O.uniform((m, k), 10 + i).astype(np.float32)
+        B = O.uniform((k, n), 20 + i).astype(np.float32)
+        dA, dB = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
+        gem_g.append((dA, dB, torch.zeros((m, n), dtype=torch.float32, device="cuda")))
+        gem_e.append((dA, dB, torch.zeros((m, n), dtype=torch.float32, device="cuda")))
+        host.append((A, B))
+    s = torch.cuda.Stream()
+    pg = phpc.Plan(phpc.F32X3, gem_g, autotune=True, stream=s)
+    pe = phpc.Plan(phpc.F32X3, gem_e)
+    junk = torch.randn(512, 512, device="cuda")
+    for rep in range(3):
+        for g in gem_g + gem_e:
+            g[2].zero_()
+        torch.cuda.synchronize()
+        with torch.cuda.stream(s):
+            pg.graph_launch(s)
+        s.synchronize()
+        junk = junk @ junk.T / 512.0
+        pe.launch()
+        torch.cuda.synchronize()
+        for (_, _, cg), (_, _, ce) in zip(gem_g, gem_e):
+            assert torch.equal(cg.view(torch.int32), ce.view(torch.int32)), rep
+    for (A, B), (_, _, dC) in zip(host, gem_e):
+        ref = A.astype(np.float64) @ B.astype(np.float64)
+        assert np.all(np.abs(dC.cpu().numpy() - ref) <= bound(A, B, np.zeros_like(ref)))
