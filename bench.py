@@ -400,6 +400,8 @@ def main():
     ap.add_argument("--no-grouped", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=1000)
+    ap.add_argument("--only", choices=["split", "fp32acc", "grouped", "sweep", "serve"],
+                    help="run one section alone and print its JSON (for profiler runs)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy three times (counter calibration)")
     args = ap.parse_args()
@@ -422,6 +424,31 @@ def main():
         return
 
     phpc = load_phpc()
+    if args.only:
+        if args.only == "split":
+            # the split CRT GEMM at N (uniform inputs), one warm-up and steps timed calls
+            A = torch.empty((args.n, args.n), dtype=torch.float64, device="cuda:0")
+            B = torch.empty_like(A)
+            C = torch.zeros_like(A)
+            phpc.fill_device(A, phpc.DIST_UNIFORM, 1)
+            phpc.fill_device(B, phpc.DIST_UNIFORM, 2)
+            info = None
+            t0 = time.perf_counter()
+            for _ in range(1 + args.steps):
+                info = phpc.dgemm_split_device(A, B, C, slices=0, target=1e-14, method=phpc.SPLIT_CRT)
+            torch.cuda.synchronize()
+            res = {"moduli": info.slices, "int_bits": info.int_bits, "calls": 1 + args.steps,
+                   "wall_s": round(time.perf_counter() - t0, 3)}
+        elif args.only == "fp32acc":
+            res = fp32_accurate_bench(phpc)
+        elif args.only == "grouped":
+            res = grouped_bench(phpc)
+        elif args.only == "sweep":
+            res = split_sweep(phpc)
+        else:
+            res = serving_bench(phpc, args.serve_requests)
+        print(json.dumps({"only": args.only, "result": res}))
+        return
     N, panel = args.n, args.panel
     p, q = (int(x) for x in args.grid.split("x"))
     dev = torch.device("cuda:0")

@@ -253,15 +253,18 @@ struct I8Cfg {
   static_assert(BK_ == 64 || BK_ == 128, "slab depth");
   static_assert(BLOCKS % NW == 0 && (BM * CPR) % 64 == 0 && (BN * CPR) % 64 == 0, "blocks must split evenly");
   static_assert(STAGES * STAGE_BYTES <= 160 * 1024, "LDS budget");
+  static_assert(STAGES >= 2 && STAGES <= 5 && 3 * PER < 64, "pipeline depth");
   static_assert(TM >= 1 && TN >= 1, "wave tile");
 };
 
-template <int CPR>
+// LDS row swizzle: chunk c of row r is stored at slot c ^ i8_swz(r). SD = 16 spreads the
+// 16-byte chunks of 16 consecutive rows over 64 banks; SD = 8 over 32 banks.
+template <int CPR, int SD = 16>
 __device__ __forceinline__ int i8_swz(int r) {
-  return (r * CPR / 16) & (CPR - 1);
+  return (r * CPR / SD) & (CPR - 1);
 }
 
-template <int BM, int BN, int WM, int WN, int BK_, int STAGES>
+template <int BM, int BN, int WM, int WN, int BK_, int STAGES, int SD = 16>
 __device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
   using K = I8Cfg<BM, BN, WM, WN, BK_, STAGES>;
   extern __shared__ __attribute__((aligned(1024))) char smem[];
@@ -291,12 +294,12 @@ __device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
 #pragma unroll
     for (int i = 0; i < K::TM; ++i) {
       const int r = wm * K::WTM + i * 32 + (lane & 31);
-      a_off[kk][i] = r * K::BK + ((c ^ i8_swz<K::CPR>(r)) << 4);
+      a_off[kk][i] = r * K::BK + ((c ^ i8_swz<K::CPR, SD>(r)) << 4);
     }
 #pragma unroll
     for (int j = 0; j < K::TN; ++j) {
       const int r = wn * K::WTN + j * 32 + (lane & 31);
-      b_off[kk][j] = K::A_BYTES + r * K::BK + ((c ^ i8_swz<K::CPR>(r)) << 4);
+      b_off[kk][j] = K::A_BYTES + r * K::BK + ((c ^ i8_swz<K::CPR, SD>(r)) << 4);
     }
   }
 
@@ -323,7 +326,7 @@ __device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
       const int blk = wave + u * K::NW;
       const bool isA = blk < K::A_BLOCKS;
       const int r = (isA ? blk : blk - K::A_BLOCKS) * K::ROWS_PER_BLOCK + lane / K::CPR;
-      const int c = (lane % K::CPR) ^ i8_swz<K::CPR>(r);
+      const int c = (lane % K::CPR) ^ i8_swz<K::CPR, SD>(r);
       const int rows = isA ? min(m_rem, BM) : min(n_rem, BN);
       const long ld = isA ? lda8 : ldb8;
       goff[u] = r < rows ? (uint32_t)((long)r * ld + c * 16) : kOOB;
@@ -352,13 +355,16 @@ __device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
     for (int s0 = 0; s0 < STAGES - 1; ++s0)
       if (s0 < nslab) issue(s0);
     for (int sidx = 0; sidx < nslab; ++sidx) {
+      // slabs issued after this one that may stay in flight
       const int later = min(STAGES - 2, nslab - 1 - sidx);
       if (later <= 0)
         wait_vm_c<0>();
       else if (later == 1)
         wait_vm_c<K::PER>();
-      else
+      else if (later == 2)
         wait_vm_c<2 * K::PER>();
+      else
+        wait_vm_c<3 * K::PER>();
       __syncthreads();
       if (sidx + STAGES - 1 < nslab) issue(sidx + STAGES - 1);
       const char *stage = smem + (sidx % STAGES) * K::STAGE_BYTES;
@@ -442,6 +448,16 @@ __global__ __launch_bounds__(256) void i8_level_256x256_w4_k128(LevelArgs a) {
   i8_level_body<256, 256, 2, 2, 128, 2>(a);
 }
 __global__ __launch_bounds__(512) void i8_level_256x128_w8(LevelArgs a) { i8_level_body<256, 128, 4, 2, 64, 4>(a); }
+__global__ __launch_bounds__(512) void i8_level_256x256_w8_s4(LevelArgs a) { i8_level_body<256, 256, 2, 4, 64, 4>(a); }
+__global__ __launch_bounds__(512) void i8_level_256x256_w8_s5(LevelArgs a) { i8_level_body<256, 256, 2, 4, 64, 5>(a); }
+__global__ __launch_bounds__(256) void i8_level_256x256_w4_s4(LevelArgs a) { i8_level_body<256, 256, 2, 2, 64, 4>(a); }
+__global__ __launch_bounds__(512) void i8_level_256x256_w8_k128_sd8(LevelArgs a) {
+  i8_level_body<256, 256, 2, 4, 128, 2, 8>(a);
+}
+__global__ __launch_bounds__(256) void i8_level_256x256_w4_k128_sd8(LevelArgs a) {
+  i8_level_body<256, 256, 2, 2, 128, 2, 8>(a);
+}
+__global__ __launch_bounds__(512) void i8_level_256x256_w8_sd8(LevelArgs a) { i8_level_body<256, 256, 2, 4, 64, 3, 8>(a); }
 
 typedef void (*LevelFn)(LevelArgs);
 struct I8Config {
@@ -458,6 +474,12 @@ const I8Config kI8Configs[] = {
     I8CFG(256, 256, 2, 4, 128, 2, i8_level_256x256_w8_k128),  // 3: 8 waves, 128-byte slabs
     I8CFG(256, 256, 2, 2, 128, 2, i8_level_256x256_w4_k128),  // 4: 4 waves, 128-byte slabs
     I8CFG(256, 128, 4, 2, 64, 4, i8_level_256x128_w8),        // 5: 8 waves, wave tile 64x64
+    I8CFG(256, 256, 2, 4, 64, 4, i8_level_256x256_w8_s4),     // 6: 8 waves, 4-stage pipeline
+    I8CFG(256, 256, 2, 4, 64, 5, i8_level_256x256_w8_s5),     // 7: 8 waves, 5-stage pipeline
+    I8CFG(256, 256, 2, 2, 64, 4, i8_level_256x256_w4_s4),     // 8: 4 waves, 4-stage pipeline
+    I8CFG(256, 256, 2, 4, 128, 2, i8_level_256x256_w8_k128_sd8),  // 9: as 3, 32-bank swizzle
+    I8CFG(256, 256, 2, 2, 128, 2, i8_level_256x256_w4_k128_sd8),  // 10: as 4, 32-bank swizzle
+    I8CFG(256, 256, 2, 4, 64, 3, i8_level_256x256_w8_sd8),        // 11: as 1, 32-bank swizzle
 };
 constexpr int kNumI8 = sizeof(kI8Configs) / sizeof(kI8Configs[0]);
 int g_i8_big = 3;
@@ -519,31 +541,50 @@ const int kModuli[kMaxMods] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227,
 struct CrtConsts {
   int L;
   int m[kMaxMods];
-  int c24[kMaxMods];           // 2^24 mod m
-  int c48[kMaxMods];           // 2^48 mod m
   int inv[kMaxMods];           // (m_1 ... m_{t-1})^-1 mod m_t
   int pmod[kMaxMods][kMaxMods];  // pmod[s][t] = (m_1 ... m_{s-1}) mod m_t, s < t
   double rm[kMaxMods];           // 1 / m, rounded
+  float rmf[kMaxMods];           // 1 / m, rounded to FP32
+  float c12[kMaxMods][6];        // 2^(12 i) mod m
 };
 
-__device__ __forceinline__ void split24(double ax, double &h, double &mid, double &lo) {
-  h = trunc(ldexp(ax, -48));
-  const double rest = ax - ldexp(h, 48);
-  mid = trunc(ldexp(rest, -24));
-  lo = rest - ldexp(mid, 24);
+// Symmetric residue of an integer-valued float 0 <= s < 2^23 (or |s| < 2^23 with the
+// correction steps below) modulo m <= 256: the quotient estimate floor(s / m) from the
+// rounded reciprocal is off by at most one (|s/m| 2^-22 < 2^-1), and q*m and s - q*m are
+// exact in FP32 (below 2^24).
+__device__ __forceinline__ int smod_f(float s, int m, float rmf) {
+  const float mf = (float)m;
+  const float q = floorf(s * rmf);
+  float r = fmaf(-q, mf, s);
+  if (r < 0.0f) r += mf;
+  if (r >= mf) r -= mf;
+  int ri = (int)r;
+  if (ri > (m - 1) / 2) ri -= m;
+  return ri;
 }
 
-// Symmetric residues of one integer-valued double x (|x| <= 2^71) for the first L
-// moduli, packed: byte t of the result array = residue modulo m_t.
+// Symmetric residues of one integer-valued double x (|x| < 2^72) for the first L
+// moduli. |x| is cut into six 12-bit chunks; sum_i chunk_i (2^(12 i) mod m) < 6 * 2^20
+// is exact in FP32, and so is its reduction mod m.
 template <int L>
 __device__ __forceinline__ void residues_of(double x, const CrtConsts &cc, int (&r)[L]) {
   const double ax = fabs(x);
-  double h, mid, lo;
-  split24(ax, h, mid, lo);
+  const double h = trunc(ldexp(ax, -48));              // < 2^24
+  const uint64_t lo = (uint64_t)(ax - ldexp(h, 48));   // exact, < 2^48
+  const uint32_t hi = (uint32_t)h;
+  float ch[6];
+  ch[0] = (float)(uint32_t)(lo & 0xFFF);
+  ch[1] = (float)(uint32_t)((lo >> 12) & 0xFFF);
+  ch[2] = (float)(uint32_t)((lo >> 24) & 0xFFF);
+  ch[3] = (float)(uint32_t)((lo >> 36) & 0xFFF);
+  ch[4] = (float)(hi & 0xFFF);
+  ch[5] = (float)(hi >> 12);
 #pragma unroll
   for (int t = 0; t < L; ++t) {
-    const double sum = h * (double)cc.c48[t] + mid * (double)cc.c24[t] + lo;  // < 2^34, exact
-    int v = smod_r(sum, cc.m[t], cc.rm[t]);
+    float sum = ch[0];
+#pragma unroll
+    for (int i = 1; i < 6; ++i) sum = fmaf(ch[i], cc.c12[t][i], sum);
+    int v = smod_f(sum, cc.m[t], cc.rmf[t]);
     if (x < 0.0) v = -v;
     if (v == 128) v = -128;  // m = 256: keep [-128, 127]
     r[t] = v;
@@ -617,11 +658,13 @@ __device__ __forceinline__ double crt_value(const int (&r)[L], const CrtConsts &
   int v[L];
 #pragma unroll
   for (int t = 0; t < L; ++t) {
+    // y = sum_s v_s (m_1..m_{s-1} mod m_t): |v_s| <= 128, factors < 256, 24-bit products
     int y = 0;
 #pragma unroll
-    for (int s2 = 0; s2 < t; ++s2) y += v[s2] * cc.pmod[s2][t];  // |y| < 20 * 128 * 256
-    // v_t = (r_t - y) * inv_t  mod m_t (|(r - y) * inv| < 2^29: exact in double)
-    v[t] = smod_r((double)(r[t] - y) * (double)cc.inv[t], cc.m[t], cc.rm[t]);
+    for (int s2 = 0; s2 < t; ++s2) y = __mul24(v[s2], cc.pmod[s2][t]) + y;  // |y| < 2^20
+    // v_t = (r_t - y) inv_t mod m_t, reduced twice in exact FP32
+    const int d = smod_f((float)(r[t] - y), cc.m[t], cc.rmf[t]);
+    v[t] = smod_f((float)(d * cc.inv[t]), cc.m[t], cc.rmf[t]);  // |d * inv| < 2^15
   }
   double hi = (double)v[L - 1], lo = 0.0;
 #pragma unroll
@@ -748,10 +791,6 @@ CrtConsts make_crt(int L) {
   for (int t = 0; t < L; ++t) {
     const int m = kModuli[t];
     c.m[t] = m;
-    long v = 1;
-    for (int b = 0; b < 24; ++b) v = v * 2 % m;
-    c.c24[t] = (int)v;
-    c.c48[t] = (int)(v * v % m);
     long prod = 1;
     for (int s2 = 0; s2 < t; ++s2) {
       c.pmod[s2][t] = (int)prod;  // (m_1..m_{s-1}) mod m_t with s = s2+1
@@ -759,6 +798,12 @@ CrtConsts make_crt(int L) {
     }
     c.inv[t] = t == 0 ? 1 : modinv((int)prod, m);
     c.rm[t] = 1.0 / (double)m;
+    c.rmf[t] = 1.0f / (float)m;
+    long w = 1;
+    for (int i = 0; i < 6; ++i) {
+      c.c12[t][i] = (float)w;
+      for (int b = 0; b < 12; ++b) w = w * 2 % m;
+    }
   }
   return c;
 }
