@@ -142,7 +142,9 @@ __device__ __forceinline__ GemmDesc uniform_desc(const GemmBatch &b, int t) {
 }
 
 template <int BM, int BN, int WM, int WN, int BK, bool MP, int VEC>
-__global__ __launch_bounds__(WM *WN * 64) void f64_gemm_kernel(GemmBatch batch) {
+// two waves per SIMD at least: a 4-wave workgroup then fits twice on a CU
+__global__ __launch_bounds__(WM *WN * 64) __attribute__((amdgpu_waves_per_eu(2))) void f64_gemm_kernel(
+    GemmBatch batch) {
   using K = F64Gemm<BM, BN, WM, WN, BK, MP, VEC>;
   __shared__ double lds[2 * K::STAGE];
 
@@ -403,7 +405,8 @@ __device__ __forceinline__ void wait_outstanding(int later, int per) {
 }
 
 template <int BM, int BN, int WM, int WN, int STAGES, bool MP>
-__global__ __launch_bounds__(WM *WN * 64) void f64_lds_kernel(GemmBatch batch) {
+__global__ __launch_bounds__(WM *WN * 64) __attribute__((amdgpu_waves_per_eu(2))) void f64_lds_kernel(
+    GemmBatch batch) {
   using K = F64Lds<BM, BN, WM, WN, STAGES, MP>;
   extern __shared__ __attribute__((aligned(1024))) char smem[];
 
@@ -639,7 +642,7 @@ const ConfigEntry kConfigs[] = {
     // 4: direct-to-LDS, 4 waves (wave tile 64x64), 4-stage pipeline; single panel only
     {128, 128, 256, 16, {0, LDS_BYTES(128, 128, 4)}, false,
      {{PHPC_V1(128, 128, 2, 2, 16, false, 1), PHPC_LDS(128, 128, 2, 2, 4, false)},
-      {PHPC_V1(128, 128, 2, 4, 16, true, 1), PHPC_LDS(128, 128, 2, 2, 4, true)}}},
+      {PHPC_V1(128, 128, 2, 2, 16, false, 1), PHPC_LDS(128, 128, 2, 2, 4, false)}}},
     // 5: direct-to-LDS, 4 waves (wave tile 32x32), 64x64 tiles, 4-stage pipeline
     {64, 64, 256, 16, {0, LDS_BYTES(64, 64, 4)}, true,
      {{PHPC_V1(64, 64, 2, 2, 16, false, 1), PHPC_LDS(64, 64, 2, 2, 4, false)},
@@ -652,6 +655,16 @@ const ConfigEntry kConfigs[] = {
     {256, 128, 512, 16, {0, LDS_BYTES(256, 128, 3)}, false,
      {{PHPC_V1(256, 128, 4, 2, 16, false, 1), PHPC_LDS(256, 128, 4, 2, 3, false)},
       {PHPC_V1(256, 128, 4, 2, 16, false, 1), PHPC_LDS(256, 128, 4, 2, 3, false)}}},
+    // 8: register-staged, 4 waves (wave tile 64x64), 128x128 tiles: 68 KiB of LDS, two
+    //    workgroups per CU so one's barrier stalls are covered by the other; single panel only
+    {128, 128, 256, 16, {0, 0}, false,
+     {{PHPC_V1(128, 128, 2, 2, 16, false, 1), PHPC_V1(128, 128, 2, 2, 16, false, 2)},
+      {PHPC_V1(128, 128, 2, 2, 16, false, 1), PHPC_V1(128, 128, 2, 2, 16, false, 2)}}},
+    // 9: direct-to-LDS, 4 waves (wave tile 64x64), 128x128 tiles, 2 stages (66 KiB, two
+    //    workgroups per CU); single panel only
+    {128, 128, 256, 16, {0, LDS_BYTES(128, 128, 2)}, false,
+     {{PHPC_V1(128, 128, 2, 2, 16, false, 1), PHPC_LDS(128, 128, 2, 2, 2, false)},
+      {PHPC_V1(128, 128, 2, 2, 16, false, 1), PHPC_LDS(128, 128, 2, 2, 2, false)}}},
 };
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 
@@ -665,10 +678,12 @@ size_t g_pinned_cap = 0;
 std::mutex g_desc_mu;  // guards the descriptor buffers below (never held with g_mu)
 hipEvent_t g_desc_done = nullptr;  // recorded after the last launch that read g_desc_dev
 
-// Note: config 4's multi-panel VEC1 fallback uses the 8-wave kernel, whose launch
-// shape differs; launch() below takes the thread count from the kernel actually used.
+// Threads per workgroup of a configuration's kernels. Single-panel configurations fill
+// their panel-grouped slots with the single-panel kernels, which are never launched with
+// panels (phpc_f64_grouped_dev rejects that), so every slot has the configuration's shape.
 int kernel_threads(int cfg, int mp, int vec2) {
-  if (cfg == 4 && mp && !vec2) return 512;
+  (void)mp;
+  (void)vec2;
   return kConfigs[cfg].threads;
 }
 

@@ -541,12 +541,12 @@ const int kModuli[kMaxMods] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227,
 struct CrtConsts {
   int L;
   int m[kMaxMods];
-  int inv[kMaxMods];           // (m_1 ... m_{t-1})^-1 mod m_t
-  int pmod[kMaxMods][kMaxMods];  // pmod[s][t] = (m_1 ... m_{s-1}) mod m_t, s < t
-  double rm[kMaxMods];           // 1 / m, rounded
+  int inv[kMaxMods];             // (m_1 ... m_{t-1})^-1 mod m_t
+  int qinv[kMaxMods][kMaxMods];  // qinv[s][t] = -(m_1 ... m_{s-1}) inv_t mod m_t, in [0, m_t), s < t
   float rmf[kMaxMods];           // 1 / m, rounded to FP32
   float c12[kMaxMods][6];        // 2^(12 i) mod m
 };
+static_assert(sizeof(CrtConsts) <= 2560, "CRT constants travel as a kernel argument");
 
 // Symmetric residue of an integer-valued float 0 <= s < 2^23 (or |s| < 2^23 with the
 // correction steps below) modulo m <= 256: the quotient estimate floor(s / m) from the
@@ -658,13 +658,13 @@ __device__ __forceinline__ double crt_value(const int (&r)[L], const CrtConsts &
   int v[L];
 #pragma unroll
   for (int t = 0; t < L; ++t) {
-    // y = sum_s v_s (m_1..m_{s-1} mod m_t): |v_s| <= 128, factors < 256, 24-bit products
-    int y = 0;
+    // v_t = (r_t - sum_s v_s pmod[s][t]) inv_t mod m_t = r_t inv_t + sum_s v_s qinv[s][t]
+    // mod m_t; |v_s| <= 128 and factors < 256 keep the sum below 2^20 (24-bit products,
+    // one exact FP32 reduction)
+    int y = __mul24(r[t], cc.inv[t]);
 #pragma unroll
-    for (int s2 = 0; s2 < t; ++s2) y = __mul24(v[s2], cc.pmod[s2][t]) + y;  // |y| < 2^20
-    // v_t = (r_t - y) inv_t mod m_t, reduced twice in exact FP32
-    const int d = smod_f((float)(r[t] - y), cc.m[t], cc.rmf[t]);
-    v[t] = smod_f((float)(d * cc.inv[t]), cc.m[t], cc.rmf[t]);  // |d * inv| < 2^15
+    for (int s2 = 0; s2 < t; ++s2) y = __mul24(v[s2], cc.qinv[s2][t]) + y;
+    v[t] = smod_f((float)y, cc.m[t], cc.rmf[t]);
   }
   double hi = (double)v[L - 1], lo = 0.0;
 #pragma unroll
@@ -791,13 +791,14 @@ CrtConsts make_crt(int L) {
   for (int t = 0; t < L; ++t) {
     const int m = kModuli[t];
     c.m[t] = m;
+    long pmod[kMaxMods];  // (m_1..m_{s-1}) mod m_t with s = s2 + 1
     long prod = 1;
     for (int s2 = 0; s2 < t; ++s2) {
-      c.pmod[s2][t] = (int)prod;  // (m_1..m_{s-1}) mod m_t with s = s2+1
+      pmod[s2] = prod;
       prod = prod * kModuli[s2] % m;
     }
     c.inv[t] = t == 0 ? 1 : modinv((int)prod, m);
-    c.rm[t] = 1.0 / (double)m;
+    for (int s2 = 0; s2 < t; ++s2) c.qinv[s2][t] = (int)((m - pmod[s2] * c.inv[t] % m) % m);
     c.rmf[t] = 1.0f / (float)m;
     long w = 1;
     for (int i = 0; i < 6; ++i) {
