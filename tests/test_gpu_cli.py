@@ -40,3 +40,23 @@ def test_cli_row_and_check(tmp_path, args, N, P, T, GW, GH):
     assert vals == [N, P, 1, GW * GH, T * T, GW * GH * T * T]
     assert float(m.group(7)) > 0 and float(m.group(8)) > 0
     assert (float(m.group(9)) > 0) == ("--no-vendor" not in args)
+
+
+def test_sweep_gathers_rows_and_process_columns(tmp_path):
+    """sweep.py runs the reference harness's testC sweep shape and appends process()'s
+    columns from a CPU-baseline file."""
+    import importlib.util
+    from conftest import PKG
+    spec = importlib.util.spec_from_file_location("sweep", PKG / "sweep.py")
+    sw = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(sw)
+    runs = [(4, 64, 4, 1, 1), (4, 128, 8, 1, 1), (1, 96, 32, 2, 2)]
+    it = {64: 0.001, 128: 0.008, 96: 0.003}
+    out = sw.sweep(runs, "size", "tc", tmp_path, it, ["--no-vendor"])
+    lines = out.read_text().splitlines()
+    assert lines[0] == sw.HEADER
+    assert [int(l.split(",")[0]) for l in lines[1:]] == [64, 96, 128]
+    for l in lines[1:]:
+        f = l.split(",")
+        assert len(f) == 14 and float(f[9]) > 0 and f[11] == ""  # no vendor pass: empty speedup_cublas
+    assert not list((tmp_path / "csv").glob("tc_*.csv"))

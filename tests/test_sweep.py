@@ -1,0 +1,32 @@
+"""The harness columns of scripts/tests.sh process() (:11-33) as sweep.py computes them,
+on hand-written rows (CPU only)."""
+import importlib.util
+
+from conftest import PKG
+
+spec = importlib.util.spec_from_file_location("sweep", PKG / "sweep.py")
+sweep = importlib.util.module_from_spec(spec)
+spec.loader.exec_module(sweep)
+
+
+def test_bc_formatting():
+    assert sweep.bc6(1, 3) == "0.333333"
+    assert sweep.bc6(10, 4) == "2.500000"
+    assert sweep.bc6(2, 3) == "0.666666"  # truncated, as bc with scale=6
+    assert sweep.bc6(0, 5) == "0"
+    assert sweep.bc6(1, 0) == ""
+
+
+def test_process_columns():
+    rows = ["1024,4,1,1,1024,1024,0.010000,0.004000,0.000000",
+            "2048,16,1,4,1024,4096,0.040000,0.016000,0.020000"]
+    out = sweep.process(rows, {1024: 1.5, 2048: 12.0})
+    assert out[0] == sweep.HEADER
+    assert out[1] == "1024,4,1,1,1024,1024,0.010000,0.004000,0.000000,150.000000,375.000000,,0.146484,0.366210"
+    assert out[2] == "2048,16,1,4,1024,4096,0.040000,0.016000,0.020000,300.000000,750.000000,600.000000,0.073242,0.183105"
+
+
+def test_reference_sweeps_listed():
+    assert set(sweep.SWEEPS) == {"test0", "testA1", "testA2", "testB", "testC", "testD"}
+    runs, key = sweep.SWEEPS["testA2"]
+    assert (4, 2048, 32, 8, 8) in runs and key == "threads"
