@@ -99,6 +99,15 @@ F64Config phpc_f64_config(int id);
 int phpc_f64_pick_config(const GemmDesc *descs, int ndesc);
 
 // Strided block pack: dst[r*ldd + c] = src[r*lds + c] for r < rows, c < cols.
+// Host <-> device copies through pinned staging buffers (rows x width bytes, strided).
+// h2d returns once the source may be reused (transfers complete in stream order); d2h
+// returns once the destination holds the data.
+int phpc_stage_h2d(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width, size_t rows,
+                   hipStream_t stream);
+int phpc_stage_d2h(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width, size_t rows,
+                   hipStream_t stream);
+// Process-wide copy and compute streams of the host-pointer entry points.
+int phpc_host_streams(hipStream_t *copy, hipStream_t *compute);
 int phpc_pack_launch(const double *src, long lds, double *dst, long ldd, int rows, int cols,
                      hipStream_t stream);
 

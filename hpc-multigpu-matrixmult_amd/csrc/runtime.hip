@@ -73,6 +73,21 @@ void do_init() {
 }
 }  // namespace
 
+int phpc_host_streams(hipStream_t *copy, hipStream_t *compute) {
+  static std::once_flag once;
+  static hipStream_t sx = nullptr, sc = nullptr;
+  static int status = PHPC_OK;
+  std::call_once(once, [] {
+    if (hipStreamCreateWithFlags(&sx, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&sc, hipStreamNonBlocking) != hipSuccess)
+      status = PHPC_ERR_HIP;
+  });
+  if (status) return phpc_set_error(status, "stream creation failed");
+  *copy = sx;
+  *compute = sc;
+  return PHPC_OK;
+}
+
 extern "C" int phpc_init(void) {
   std::call_once(g_init_once, do_init);
   if (g_init_status != PHPC_OK)
@@ -219,12 +234,13 @@ static int gemm_cuda_impl(const double *a, int lda, const double *b, int ldb, do
   double *dB = (double *)phpc_scratch((sb ? sb : 1) * 8, 1);
   double *dC = (double *)phpc_scratch(sc * 8, 2);
   if (!dA || !dB || !dC) return phpc_set_error(PHPC_ERR_MEM, "device allocation failed");
-  hipStream_t s = nullptr;
+  hipStream_t s, sc_unused;
+  if (int rc = phpc_host_streams(&s, &sc_unused)) return rc;
   if (k > 0) {
-    PHPC_HIP(hipMemcpy2DAsync(dA, (size_t)k * 8, a, (size_t)lda * 8, (size_t)k * 8, m, hipMemcpyHostToDevice, s));
-    PHPC_HIP(hipMemcpy2DAsync(dB, (size_t)n * 8, b, (size_t)ldb * 8, (size_t)n * 8, k, hipMemcpyHostToDevice, s));
+    if (int rc = phpc_stage_h2d(dA, (size_t)k * 8, a, (size_t)lda * 8, (size_t)k * 8, m, s)) return rc;
+    if (int rc = phpc_stage_h2d(dB, (size_t)n * 8, b, (size_t)ldb * 8, (size_t)n * 8, k, s)) return rc;
   }
-  PHPC_HIP(hipMemcpy2DAsync(dC, (size_t)n * 8, c, (size_t)ldc * 8, (size_t)n * 8, m, hipMemcpyHostToDevice, s));
+  if (int rc = phpc_stage_h2d(dC, (size_t)n * 8, c, (size_t)ldc * 8, (size_t)n * 8, m, s)) return rc;
   hipEvent_t e0, e1;
   PHPC_HIP(hipEventCreate(&e0));
   PHPC_HIP(hipEventCreate(&e1));
@@ -240,8 +256,7 @@ static int gemm_cuda_impl(const double *a, int lda, const double *b, int ldb, do
     }
   }
   PHPC_HIP(hipEventRecord(e1, s));
-  if (rc == PHPC_OK)
-    PHPC_HIP(hipMemcpy2DAsync(c, (size_t)ldc * 8, dC, (size_t)n * 8, (size_t)n * 8, m, hipMemcpyDeviceToHost, s));
+  if (rc == PHPC_OK) rc = phpc_stage_d2h(c, (size_t)ldc * 8, dC, (size_t)n * 8, (size_t)n * 8, m, s);
   PHPC_HIP(hipStreamSynchronize(s));
   float ms = 0;
   PHPC_HIP(hipEventElapsedTime(&ms, e0, e1));

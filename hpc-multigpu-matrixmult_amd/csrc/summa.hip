@@ -23,6 +23,7 @@
 #include "phpc_internal.h"
 #include "phpc_summa.h"
 
+#include <algorithm>
 #include <vector>
 
 namespace {
@@ -156,34 +157,104 @@ extern "C" int phpc_summa_device(int p, int q, int panel, int mode, const double
   return phpc_set_error(PHPC_OK, nullptr);
 }
 
+// Host pointers. FUSED and CHAINED: every output's arithmetic depends only on its own
+// row and column and the panel width, so C can be computed in row bands. B goes over
+// first; then the A and C rows of band i+1 travel (pinned staging, copy stream) while band
+// i computes (compute stream) and band i-1 comes back. Other modes copy everything in,
+// run phpc_summa_device and copy C out.
 extern "C" int phpc_summa_host(int p, int q, int panel, int mode, const double *A, const double *B, double *C, int N,
                                int wg, float *kernel_seconds) {
   if (kernel_seconds) *kernel_seconds = 0.0f;
   if (phpc_init()) return phpc_last_error();
   if (N < 0 || (N > 0 && (!A || !B || !C))) return phpc_set_error(PHPC_ERR_ARG, "bad SUMMA host arguments");
+  if (p < 1 || q < 1 || p > N || q > N) return phpc_set_error(PHPC_ERR_ARG, "bad grid %dx%d for N=%d", p, q, N);
   if (N == 0) return phpc_set_error(PHPC_OK, nullptr);
-  const size_t bytes = (size_t)N * N * 8;
+  const size_t bytes = (size_t)N * N * 8, row = (size_t)N * 8;
   double *dA = (double *)phpc_scratch(bytes, 5);
   double *dB = (double *)phpc_scratch(bytes, 6);
   double *dC = (double *)phpc_scratch(bytes, 7);
   if (!dA || !dB || !dC) return phpc_set_error(PHPC_ERR_MEM, "device allocation failed (3 x %zu bytes)", bytes);
-  hipStream_t s = nullptr;
-  PHPC_HIP(hipMemcpyAsync(dA, A, bytes, hipMemcpyHostToDevice, s));
-  PHPC_HIP(hipMemcpyAsync(dB, B, bytes, hipMemcpyHostToDevice, s));
-  PHPC_HIP(hipMemcpyAsync(dC, C, bytes, hipMemcpyHostToDevice, s));
-  hipEvent_t e0, e1;
-  PHPC_HIP(hipEventCreate(&e0));
-  PHPC_HIP(hipEventCreate(&e1));
-  PHPC_HIP(hipEventRecord(e0, s));
-  int rc = phpc_summa_device(p, q, panel, mode, dA, dB, dC, N, wg, s);
-  PHPC_HIP(hipEventRecord(e1, s));
-  if (rc == PHPC_OK) PHPC_HIP(hipMemcpyAsync(C, dC, bytes, hipMemcpyDeviceToHost, s));
-  PHPC_HIP(hipStreamSynchronize(s));
-  float ms = 0;
-  PHPC_HIP(hipEventElapsedTime(&ms, e0, e1));
-  PHPC_HIP(hipEventDestroy(e0));
-  PHPC_HIP(hipEventDestroy(e1));
+  hipStream_t sx, sc;
+  if (int rc = phpc_host_streams(&sx, &sc)) return rc;
+  const bool banded = mode == PHPC_SUMMA_FUSED || mode == PHPC_SUMMA_CHAINED;
+  if (!banded) {
+    if (int rc = phpc_stage_h2d(dA, row, A, row, row, N, sx)) return rc;
+    if (int rc = phpc_stage_h2d(dB, row, B, row, row, N, sx)) return rc;
+    if (int rc = phpc_stage_h2d(dC, row, C, row, row, N, sx)) return rc;
+    hipEvent_t e0, e1;
+    PHPC_HIP(hipEventCreate(&e0));
+    PHPC_HIP(hipEventCreate(&e1));
+    PHPC_HIP(hipEventRecord(e0, sx));
+    int rc = phpc_summa_device(p, q, panel, mode, dA, dB, dC, N, wg, sx);
+    PHPC_HIP(hipEventRecord(e1, sx));
+    if (rc == PHPC_OK) rc = phpc_stage_d2h(C, row, dC, row, row, N, sx);
+    PHPC_HIP(hipStreamSynchronize(sx));
+    float ms = 0;
+    PHPC_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    if (kernel_seconds) *kernel_seconds = ms / 1000.0f;
+    return phpc_set_error(PHPC_OK, nullptr);
+  }
+  const int lcm = p / gcd(p, q) * q;
+  const int w = panel > 0 ? panel : N / lcm;
+  if (w < 1) return phpc_set_error(PHPC_ERR_ARG, "panel width N/lcm(p,q) = 0 for N=%d, grid %dx%d", N, p, q);
+  const LaunchPolicy &pol = phpc_policy();
+  const int G = wg > 0 ? wg : pol.grid_override;
+  // bands of whole 256-row tiles, about 16 of them
+  const int hb = std::max(256, (N / 16 + 255) / 256 * 256);
+  const int nb = (N + hb - 1) / hb;
+  std::vector<hipEvent_t> ev_in(nb), ev_s(nb), ev_e(nb);
+  for (int i = 0; i < nb; ++i) {
+    PHPC_HIP(hipEventCreateWithFlags(&ev_in[i], hipEventDisableTiming));
+    PHPC_HIP(hipEventCreate(&ev_s[i]));
+    PHPC_HIP(hipEventCreate(&ev_e[i]));
+  }
+  int rc = phpc_stage_h2d(dB, row, B, row, row, N, sx);
+  for (int i = 0; i <= nb && rc == PHPC_OK; ++i) {
+    if (i < nb) {
+      const long r0 = (long)i * hb;
+      const int rows = (int)std::min<long>(hb, N - r0);
+      rc = phpc_stage_h2d(dA + r0 * N, row, A + r0 * N, row, row, rows, sx);
+      if (!rc) rc = phpc_stage_h2d(dC + r0 * N, row, C + r0 * N, row, row, rows, sx);
+      if (rc) break;
+      PHPC_HIP(hipEventRecord(ev_in[i], sx));
+      PHPC_HIP(hipStreamWaitEvent(sc, ev_in[i], 0));
+      GemmDesc d{};
+      d.A = dA + r0 * N;
+      d.B = dB;
+      d.C = dC + r0 * N;
+      d.lda = N;
+      d.ldb = N;
+      d.ldc = N;
+      d.m = rows;
+      d.n = N;
+      d.k = N;
+      d.panel = mode == PHPC_SUMMA_CHAINED ? 0 : w;
+      PHPC_HIP(hipEventRecord(ev_s[i], sc));
+      rc = phpc_f64_grouped(&d, 1, G, pol.config_id, sc);
+      PHPC_HIP(hipEventRecord(ev_e[i], sc));
+      if (rc) break;
+    }
+    if (i >= 1) {  // band i-1 back to the host once computed
+      const long r0 = (long)(i - 1) * hb;
+      const int rows = (int)std::min<long>(hb, N - r0);
+      PHPC_HIP(hipStreamWaitEvent(sx, ev_e[i - 1], 0));
+      rc = phpc_stage_d2h(C + r0 * N, row, dC + r0 * N, row, row, rows, sx);
+    }
+  }
+  (void)hipStreamSynchronize(sc);
+  (void)hipStreamSynchronize(sx);
+  float total = 0;
+  for (int i = 0; i < nb; ++i) {
+    float ms = 0;
+    if (rc == PHPC_OK && hipEventElapsedTime(&ms, ev_s[i], ev_e[i]) == hipSuccess) total += ms;
+    (void)hipEventDestroy(ev_in[i]);
+    (void)hipEventDestroy(ev_s[i]);
+    (void)hipEventDestroy(ev_e[i]);
+  }
   if (rc) return rc;
-  if (kernel_seconds) *kernel_seconds = ms / 1000.0f;
+  if (kernel_seconds) *kernel_seconds = total / 1000.0f;
   return phpc_set_error(PHPC_OK, nullptr);
 }
