@@ -532,7 +532,15 @@ extern "C" int phpc_sgemm_device(const float *dA, long lda, const float *dB, lon
   d.k = k;
   const bool vec4 = phpc_f32_vec4_ok(&d, 1);
   int rc;
-  if (method == PHPC_SGEMM_BF16X3) {
+  if (method == PHPC_SGEMM_BF16X3 && (long)m * n >= 2048L * 2048 && k >= 256) {
+    // large products: slice planes cut once, then the three slice levels on the
+    // direct-to-LDS pipeline
+    static SharedWorkspace ws;
+    std::lock_guard<std::mutex> lk(ws.mu);
+    ws.after_previous((hipStream_t)stream);
+    rc = phpc_sgemm_bf16x3_planes(dA, lda, dB, ldb, dC, ldc, m, k, n, (hipStream_t)stream, &ws.ws);
+    ws.mark_used((hipStream_t)stream);
+  } else if (method == PHPC_SGEMM_BF16X3) {
     const int tiles = phpc_f32_plan_tiles(&d, 1, 2);
     rc = phpc_f32x3_launch(nullptr, &d, 1, tiles, vec4, 0, (hipStream_t)stream);
   } else {

@@ -2,6 +2,8 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <mutex>
 #include <stdint.h>
 #include <stddef.h>
 
@@ -126,9 +128,28 @@ struct Workspace {
   ~Workspace() { release(); }
 };
 
+// A workspace shared by calls that may come on different streams: the holder of the
+// lock orders its stream after the previous holder's last use (device-side wait), and
+// marks its own last use before unlocking.
+struct SharedWorkspace {
+  std::mutex mu;
+  Workspace ws;
+  hipEvent_t last = nullptr;
+  void after_previous(hipStream_t stream) {
+    if (last) (void)hipStreamWaitEvent(stream, last, 0);
+  }
+  void mark_used(hipStream_t stream) {
+    if (!last) (void)hipEventCreateWithFlags(&last, hipEventDisableTiming);
+    if (last) (void)hipEventRecord(last, stream);
+  }
+};
+
 #include "phpc_split.h"
 // Split-precision GEMM with an explicit workspace and method (the public entry point
 // uses a process-wide workspace and the method set by phpc_split_set_method).
+// FP32-accurate 3 x BF16 GEMM through BF16 slice planes (workspace slots 0 and 1).
+int phpc_sgemm_bf16x3_planes(const float *dA, long lda, const float *dB, long ldb, float *dC, long ldc, int m, int k,
+                             int n, hipStream_t s, Workspace *ws);
 int phpc_dgemm_split_ws(const double *dA, long lda, const double *dB, long ldb, double *dC, long ldc, int m, int k,
                         int n, int slices, double target_err, int method, hipStream_t stream, Workspace *ws,
                         phpc_split_info *info);

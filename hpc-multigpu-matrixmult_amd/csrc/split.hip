@@ -25,12 +25,15 @@
 #include <algorithm>
 #include <cmath>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 namespace {
 
 typedef int i4v __attribute__((ext_vector_type(4)));
 typedef int i16v __attribute__((ext_vector_type(16)));
+typedef float f16v __attribute__((ext_vector_type(16)));
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 typedef __amdgpu_buffer_rsrc_t rsrc_t;
 typedef __attribute__((address_space(3))) void lds_void;
 typedef double __attribute__((address_space(1))) gdouble;
@@ -214,6 +217,9 @@ struct LevelArgs {
   int mod_m;  // 0: level mode (FP64 epilogue above)
   int8_t *res;
   long ldres;
+  // BF16 mode (operands are BF16 slices): C32[i,j] += the FP32 sum
+  float *C32;
+  long ldc32;
 };
 
 // x mod m in the symmetric range (m odd: |r| <= (m-1)/2; m = 256: [-128, 127]) for
@@ -264,7 +270,10 @@ __device__ __forceinline__ int i8_swz(int r) {
   return (r * CPR / SD) & (CPR - 1);
 }
 
-template <int BM, int BN, int WM, int WN, int BK_, int STAGES, int SD = 16>
+// BF = false: int8 operands, v_mfma_i32_32x32x32_i8, int32 sums (level or residue
+// epilogue). BF = true: BF16 operands (8 per 16-byte chunk), v_mfma_f32_32x32x16_bf16,
+// FP32 sums added to C32. The byte layout and the pipeline are the same.
+template <int BM, int BN, int WM, int WN, int BK_, int STAGES, int SD = 16, bool BF = false>
 __device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
   using K = I8Cfg<BM, BN, WM, WN, BK_, STAGES>;
   extern __shared__ __attribute__((aligned(1024))) char smem[];
@@ -344,11 +353,12 @@ __device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
       }
     };
 
-    i16v acc[K::TM][K::TN];
+    using acc_t = typename std::conditional<BF, f16v, i16v>::type;
+    acc_t acc[K::TM][K::TN];
 #pragma unroll
     for (int i = 0; i < K::TM; ++i)
 #pragma unroll
-      for (int j = 0; j < K::TN; ++j) acc[i][j] = i16v{};
+      for (int j = 0; j < K::TN; ++j) acc[i][j] = acc_t{};
 
     __syncthreads();
 #pragma unroll
@@ -379,11 +389,34 @@ __device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
         for (int i = 0; i < K::TM; ++i)
 #pragma unroll
           for (int j = 0; j < K::TN; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[i], b[j], acc[i][j], 0, 0, 0);
+            if constexpr (BF)
+              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8v, a[i]),
+                                                                 __builtin_bit_cast(bf8v, b[j]), acc[i][j], 0, 0, 0);
+            else
+              acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[i], b[j], acc[i][j], 0, 0, 0);
       }
     }
 
     // epilogue: 32x32 block, lane l / reg r -> row 8(r/4) + 4(l/32) + r%4, column l%32
+    if constexpr (BF) {
+      float __attribute__((address_space(1))) *Cg = (float __attribute__((address_space(1))) *)uni(args.C32);
+      const long ldc32 = uni(args.ldc32);
+#pragma unroll
+      for (int i = 0; i < K::TM; ++i)
+#pragma unroll
+        for (int j = 0; j < K::TN; ++j) {
+          const int col = col0 + wn * K::WTN + j * 32 + (lane & 31);
+          if (col >= n) continue;
+#pragma unroll
+          for (int r = 0; r < 16; ++r) {
+            const int row = row0 + wm * K::WTM + i * 32 + 8 * (r / 4) + 4 * (lane >> 5) + (r % 4);
+            if (row >= m) continue;
+            auto *pc = Cg + (long)row * ldc32 + col;
+            *pc = *pc + acc[i][j][r];
+          }
+        }
+      continue;
+    }
     const int mod_m = uni(args.mod_m);
     if (mod_m > 0) {
       const int firstr = uni(args.first);
@@ -458,6 +491,13 @@ __global__ __launch_bounds__(256) void i8_level_256x256_w4_k128_sd8(LevelArgs a)
   i8_level_body<256, 256, 2, 2, 128, 2, 8>(a);
 }
 __global__ __launch_bounds__(512) void i8_level_256x256_w8_sd8(LevelArgs a) { i8_level_body<256, 256, 2, 4, 64, 3, 8>(a); }
+// BF16 slices on the same pipeline (FP32-accurate 3 x BF16 GEMM)
+__global__ __launch_bounds__(512) void bf16_level_256x256_w8_k128(LevelArgs a) {
+  i8_level_body<256, 256, 2, 4, 128, 2, 16, true>(a);
+}
+__global__ __launch_bounds__(256) void bf16_level_128x128_w4(LevelArgs a) {
+  i8_level_body<128, 128, 2, 2, 64, 4, 16, true>(a);
+}
 
 typedef void (*LevelFn)(LevelArgs);
 struct I8Config {
@@ -484,32 +524,35 @@ const I8Config kI8Configs[] = {
 constexpr int kNumI8 = sizeof(kI8Configs) / sizeof(kI8Configs[0]);
 int g_i8_big = 3;
 
+// BF16-slice level kernels (FP32-accurate 3 x BF16 GEMM), kept apart from the int8
+// table so that phpc_split_set_config can only pick int8 kernels
+const I8Config kBfConfigs[] = {
+    I8CFG(256, 256, 2, 4, 128, 2, bf16_level_256x256_w8_k128),  // 0: large problems
+    I8CFG(128, 128, 2, 2, 64, 4, bf16_level_128x128_w4),        // 1: small problems
+};
+constexpr int kNumBf = sizeof(kBfConfigs) / sizeof(kBfConfigs[0]);
+
 std::mutex g_mu;
 bool g_ready = false;
 int g_occ[kNumI8];
+int g_occ_bf[kNumBf];
+
+int occupancy_of(const I8Config &c) {
+  const void *fn = reinterpret_cast<const void *>(c.fn);
+  (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.lds);
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, c.threads, c.lds) != hipSuccess || n < 1) n = 1;
+  return n;
+}
 
 void init_locked() {
   if (g_ready) return;
-  for (int c = 0; c < kNumI8; ++c) {
-    const void *fn = reinterpret_cast<const void *>(kI8Configs[c].fn);
-    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kI8Configs[c].lds);
-    int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, kI8Configs[c].threads, kI8Configs[c].lds) != hipSuccess ||
-        n < 1)
-      n = 1;
-    g_occ[c] = n;
-  }
+  for (int c = 0; c < kNumI8; ++c) g_occ[c] = occupancy_of(kI8Configs[c]);
+  for (int c = 0; c < kNumBf; ++c) g_occ_bf[c] = occupancy_of(kBfConfigs[c]);
   g_ready = true;
 }
 
-int launch_level(const LevelArgs &a, int cfg, hipStream_t s) {
-  const I8Config &c = kI8Configs[cfg];
-  int occ;
-  {
-    std::lock_guard<std::mutex> lk(g_mu);
-    init_locked();
-    occ = g_occ[cfg];
-  }
+int launch_with(const I8Config &c, int occ, const LevelArgs &a, hipStream_t s) {
   const long tiles = (long)((a.m + c.bm - 1) / c.bm) * ((a.n + c.bn - 1) / c.bn);
   long G = (long)occ * phpc_device().cus;
   if (G > tiles) G = tiles;
@@ -517,6 +560,100 @@ int launch_level(const LevelArgs &a, int cfg, hipStream_t s) {
   hipLaunchKernelGGL(c.fn, dim3((unsigned)G), dim3(c.threads), c.lds, s, a);
   PHPC_HIP(hipGetLastError());
   return PHPC_OK;
+}
+
+int launch_level(const LevelArgs &a, int cfg, hipStream_t s) {
+  int occ;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    init_locked();
+    occ = g_occ[cfg];
+  }
+  return launch_with(kI8Configs[cfg], occ, a, s);
+}
+
+int launch_bf_level(const LevelArgs &a, int cfg, hipStream_t s) {
+  int occ;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    init_locked();
+    occ = g_occ_bf[cfg];
+  }
+  return launch_with(kBfConfigs[cfg], occ, a, s);
+}
+
+// ---------------------------------------------------------------------------
+// FP32 -> three BF16 slice planes for the level kernels: x = hi + mid + lo exactly,
+// each rounded to nearest BF16 (the same cut as the FP32-accurate grouped kernel)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t bf16_rne(float x) {
+  const uint32_t u = __float_as_uint(x);
+  return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
+__device__ __forceinline__ void split3(float x, uint32_t &h, uint32_t &mi, uint32_t &lo) {
+  h = bf16_rne(x);
+  const float r1 = x - __uint_as_float(h << 16);
+  mi = bf16_rne(r1);
+  const float r2 = r1 - __uint_as_float(mi << 16);
+  lo = bf16_rne(r2);
+}
+typedef unsigned int u4v __attribute__((ext_vector_type(4)));
+
+// A3 row i = [hi | mid | lo], each plane kp BF16 wide (zero beyond k); 8 k per thread
+__global__ __launch_bounds__(256) void bf16x3_rows_kernel(const float *__restrict__ A, long lda, int m, int k,
+                                                          long kp, uint16_t *__restrict__ A3) {
+  const int i = blockIdx.y;
+  const float *row = A + (long)i * lda;
+  uint16_t *out = A3 + (long)i * 3 * kp;
+  for (long c = ((long)blockIdx.x * blockDim.x + threadIdx.x) * 8; c < kp; c += (long)gridDim.x * blockDim.x * 8) {
+    uint32_t h[8], mi[8], lo[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) split3(c + e < k ? row[c + e] : 0.0f, h[e], mi[e], lo[e]);
+    u4v vh, vm, vl;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      vh[q] = h[2 * q] | (h[2 * q + 1] << 16);
+      vm[q] = mi[2 * q] | (mi[2 * q + 1] << 16);
+      vl[q] = lo[2 * q] | (lo[2 * q + 1] << 16);
+    }
+    *reinterpret_cast<u4v *>(out + c) = vh;
+    *reinterpret_cast<u4v *>(out + kp + c) = vm;
+    *reinterpret_cast<u4v *>(out + 2 * kp + c) = vl;
+  }
+}
+
+// B3 row j (column j of B) = [lo | mid | hi], through a 64 x 64 LDS transpose
+__global__ __launch_bounds__(256) void bf16x3_cols_kernel(const float *__restrict__ B, long ldb, int k, int n,
+                                                          long kp, uint16_t *__restrict__ B3) {
+  __shared__ float tile[64][65];
+  const int k0 = blockIdx.x * 64, j0 = blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int r = ty; r < 64; r += 4) {
+    const int kk = k0 + r, jj = j0 + tx;
+    tile[r][tx] = (kk < k && jj < n) ? B[(long)kk * ldb + jj] : 0.0f;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    const int item = threadIdx.x + it * 256;  // 64 columns x 8 groups of 8 k
+    const int c = item >> 3, kq = (item & 7) * 8;
+    const int jj = j0 + c;
+    if (jj >= n) continue;
+    uint32_t h[8], mi[8], lo[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) split3(tile[kq + e][c], h[e], mi[e], lo[e]);
+    u4v vh, vm, vl;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      vh[q] = h[2 * q] | (h[2 * q + 1] << 16);
+      vm[q] = mi[2 * q] | (mi[2 * q + 1] << 16);
+      vl[q] = lo[2 * q] | (lo[2 * q + 1] << 16);
+    }
+    uint16_t *out = B3 + (long)jj * 3 * kp + k0 + kq;
+    *reinterpret_cast<u4v *>(out) = vl;
+    *reinterpret_cast<u4v *>(out + kp) = vm;
+    *reinterpret_cast<u4v *>(out + 2 * kp) = vh;
+  }
 }
 
 // Error estimate used to pick the slice count: the truncated terms of one product are
@@ -847,18 +984,21 @@ extern "C" double phpc_split_error_estimate(int slices, double ratio_a, double r
 }
 
 namespace {
-std::mutex g_default_ws_mu;
-Workspace g_default_ws;
+SharedWorkspace g_default_ws;
 }  // namespace
 
 extern "C" int phpc_dgemm_split_device(const double *dA, long lda, const double *dB, long ldb, double *dC, long ldc,
                                        int m, int k, int n, int slices, double target_err, void *stream_,
                                        phpc_split_info *info) {
-  // the process-wide workspace serves one call at a time
-  std::lock_guard<std::mutex> lk(g_default_ws_mu);
-  return phpc_dgemm_split_ws(dA, lda, dB, ldb, dC, ldc, m, k, n, slices, target_err,
-                             method_crt ? PHPC_SPLIT_CRT : PHPC_SPLIT_SLICES, (hipStream_t)stream_, &g_default_ws,
-                             info);
+  // the process-wide workspace serves one call at a time, each on its stream after the
+  // previous call's use
+  std::lock_guard<std::mutex> lk(g_default_ws.mu);
+  hipStream_t s = (hipStream_t)stream_;
+  g_default_ws.after_previous(s);
+  const int rc = phpc_dgemm_split_ws(dA, lda, dB, ldb, dC, ldc, m, k, n, slices, target_err,
+                                     method_crt ? PHPC_SPLIT_CRT : PHPC_SPLIT_SLICES, s, &g_default_ws.ws, info);
+  g_default_ws.mark_used(s);
+  return rc;
 }
 
 int phpc_dgemm_split_ws(const double *dA, long lda, const double *dB, long ldb, double *dC, long ldc, int m, int k,
@@ -1065,4 +1205,45 @@ int phpc_dgemm_split_ws(const double *dA, long lda, const double *dB, long ldb, 
     info->int_bits = 7 * s_use;
   }
   return phpc_set_error(PHPC_OK, nullptr);
+}
+
+// FP32-accurate GEMM from BF16 slice planes: C += A*B (FP32, row-major, device pointers).
+// A3 rows hold [hi | mid | lo] and B3 rows (columns of B) [lo | mid | hi], so the six
+// products with slice-index sum <= 2 form three contiguous k ranges, run smallest first:
+// A3[:, 0:3kp] . B3[:, 0:3kp] = hi.lo + mid.mid + lo.hi, A3[:, 0:2kp] . B3[:, kp:3kp] =
+// hi.mid + mid.hi, A3[:, 0:kp] . B3[:, 2kp:3kp] = hi.hi, each adding its FP32 sums to C.
+int phpc_sgemm_bf16x3_planes(const float *dA, long lda, const float *dB, long ldb, float *dC, long ldc, int m, int k,
+                             int n, hipStream_t s, Workspace *ws) {
+  const long kp = ((long)k + 63) / 64 * 64;
+  if (3 * kp * 2 >= 0x7fffffffL / 256) return phpc_set_error(PHPC_ERR_ARG, "k = %d too large for the slice planes", k);
+  uint16_t *A3 = (uint16_t *)ws->get((size_t)m * 3 * kp * 2, 0, s);
+  uint16_t *B3 = (uint16_t *)ws->get((size_t)n * 3 * kp * 2, 1, s);
+  if (!A3 || !B3) return phpc_set_error(PHPC_ERR_MEM, "slice plane allocation failed");
+  {
+    const long per_row = (kp / 8 + 255) / 256;
+    hipLaunchKernelGGL(bf16x3_rows_kernel, dim3((unsigned)std::min<long>(per_row, 64), (unsigned)m), dim3(256), 0, s,
+                       dA, lda, m, k, kp, A3);
+    PHPC_HIP(hipGetLastError());
+    hipLaunchKernelGGL(bf16x3_cols_kernel, dim3((unsigned)(kp / 64), (unsigned)((n + 63) / 64)), dim3(256), 0, s, dB,
+                       ldb, k, n, kp, B3);
+    PHPC_HIP(hipGetLastError());
+  }
+  const long tiles256 = (long)((m + 255) / 256) * ((n + 255) / 256);
+  const int cfg = tiles256 >= (long)phpc_device().cus / 2 ? 0 : 1;
+  for (int lev = 3; lev >= 1; --lev) {
+    LevelArgs a{};
+    a.A8 = (const int8_t *)A3;
+    a.B8 = (const int8_t *)B3;
+    a.lda8 = 3 * kp * 2;
+    a.ldb8 = 3 * kp * 2;
+    a.a0 = 0;
+    a.b0 = (3 - lev) * kp * 2;
+    a.m = m;
+    a.n = n;
+    a.kl = (int)(lev * kp * 2);
+    a.C32 = dC;
+    a.ldc32 = ldc;
+    if (int rc = launch_bf_level(a, cfg, s)) return rc;
+  }
+  return PHPC_OK;
 }

@@ -95,3 +95,36 @@ def test_bf16x3_plan_graph_replays_equal_eager():
     for (A, B), (_, _, dC) in zip(host, gem_e):
         ref = A.astype(np.float64) @ B.astype(np.float64)
         assert np.all(np.abs(dC.cpu().numpy() - ref) <= bound(A, B, np.zeros_like(ref)))
+
+
+@pytest.mark.parametrize("m,n,k", [(2100, 2300, 1000), (2048, 2048, 4096), (4096, 2048, 257)])
+def test_bf16x3_slice_planes_within_fp32_bound(m, n, k):
+    """Large products take the slice-plane path (BF16 planes cut once, three slice
+    levels on the direct-to-LDS pipeline); ragged shapes included."""
+    import torch
+    A = O.uniform((m, k), 7).astype(np.float32)
+    B = O.uniform((k, n), 8).astype(np.float32)
+    C0 = O.uniform((m, n), 9).astype(np.float32)
+    dC = torch.from_numpy(C0.copy()).cuda()
+    phpc.sgemm_device(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(), dC, method=phpc.SGEMM_BF16X3)
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(1)
+    r, c = rng.integers(0, m, 512), rng.integers(0, n, 512)
+    A64, B64 = A.astype(np.float64), B.astype(np.float64)
+    ref = C0[r, c].astype(np.float64) + np.einsum("ik,ki->i", A64[r], B64[:, c])
+    mag = np.abs(C0[r, c]).astype(np.float64) + np.einsum("ik,ki->i", np.abs(A64[r]), np.abs(B64[:, c]))
+    err = np.abs(dC.cpu().numpy()[r, c].astype(np.float64) - ref)
+    assert np.all(err <= ((k + 2) * 2.0 ** -24 + 2.0 ** -23) * mag)
+
+
+def test_bf16x3_slice_planes_exact_on_small_integers():
+    import torch
+    rng = np.random.default_rng(6)
+    m, n, k = 2048, 2304, 640
+    A = rng.integers(-8, 9, (m, k)).astype(np.float32)
+    B = rng.integers(-8, 9, (k, n)).astype(np.float32)
+    dC = torch.zeros((m, n), dtype=torch.float32, device="cuda")
+    phpc.sgemm_device(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(), dC, method=phpc.SGEMM_BF16X3)
+    torch.cuda.synchronize()
+    want = (A.astype(np.int64) @ B.astype(np.int64)).astype(np.float32)
+    assert np.array_equal(dC.cpu().numpy(), want)
