@@ -233,6 +233,12 @@ void phpc_server::process(Worker *w, std::vector<Req *> &batch) {
     if (nb > 1 && hipMemcpyAsync(w->desc_dev, gd, sizeof(GemmDesc) * nb, hipMemcpyHostToDevice, s) != hipSuccess)
       status = PHPC_ERR_HIP;
     if (status == PHPC_OK) status = phpc_f64_grouped_dev((const GemmDesc *)w->desc_dev, gd, nb, 0, cfg, s);
+  } else if (status == PHPC_OK && !batch[0]->small && (long)batch[0]->r.m * batch[0]->r.n >= 2048L * 2048 &&
+             batch[0]->r.k >= 256) {
+    // a large FP32-class request: FP32-accurate 3 x BF16 slice planes, alone
+    const phpc_request &r = batch[0]->r;
+    status = phpc_sgemm_bf16x3_planes((const float *)ptr(batch[0], 0), r.k, (const float *)ptr(batch[0], 1), r.n,
+                                      (float *)ptr(batch[0], 2), r.n, r.m, r.k, r.n, s, &w->ws);
   } else if (status == PHPC_OK) {
     F32Desc *fd = (F32Desc *)w->desc_host;
     long work = 0;

@@ -25,7 +25,9 @@ extern "C" {
  * min(m, n, k) >= the split threshold (phpc_server_set_split_min, default 4096), else the
  * plain FP64 MFMA kernel, batched with other such requests. */
 #define PHPC_ACC_FP64 0
-/* FP32 inputs, FP32 MFMA grouped GEMM, batched. */
+/* FP32 inputs, FP32-level result: the FP32 MFMA grouped GEMM, batched; requests that fill
+ * the GPU alone with m*n >= 2048^2 and k >= 256 use the FP32-accurate 3 x BF16 slice
+ * planes (phpc_sgemm_device, PHPC_SGEMM_BF16X3). */
 #define PHPC_ACC_FP32 1
 
 typedef struct phpc_request {
