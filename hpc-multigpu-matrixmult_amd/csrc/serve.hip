@@ -131,7 +131,8 @@ struct phpc_server {
   std::unordered_map<uint64_t, Req *> all;
   uint64_t next_id = 1;
   int pending = 0;
-  int big_running = 0;  // requests that fill the GPU alone, in flight (at most one)
+  int big_running = 0;  // requests that fill the GPU alone, in flight
+  int big_limit = 1;    // at most this many of them at once
   bool stop = false;
   std::vector<Worker *> workers;
 
@@ -144,15 +145,15 @@ struct phpc_server {
 
 // The next request to run, or queue.end() if none may start now: the oldest (FIFO), or
 // the one with the least estimated device time less its waiting time divided by the
-// aging factor (shortest first). A request that fills the GPU alone does not start
-// while another one runs (two of them would only share the GPU and both finish late);
-// small requests may start beside it.
+// aging factor (shortest first). Requests that fill the GPU alone start only while fewer
+// than big_limit of them run (each one beyond the first only shares the GPU); small
+// requests may start beside them.
 std::deque<Req *>::iterator phpc_server::pick_locked() {
   const clk::time_point now = clk::now();
   auto best = queue.end();
   double best_key = 0.0;
   for (auto it = queue.begin(); it != queue.end(); ++it) {
-    if (big_running > 0 && !(*it)->small) continue;
+    if (big_running >= big_limit && !(*it)->small) continue;
     if (policy == PHPC_SCHED_FIFO) return it;
     double key = (*it)->est;
     if (aging > 0.0) key -= std::chrono::duration<double>(now - (*it)->arrive).count() / aging;
@@ -463,6 +464,16 @@ extern "C" int phpc_server_set_policy(phpc_server_t srv, int policy, double agin
   std::lock_guard<std::mutex> lk(srv->mu);
   srv->policy = policy;
   srv->aging = aging;
+  return phpc_set_error(PHPC_OK, nullptr);
+}
+
+extern "C" int phpc_server_set_big_limit(phpc_server_t srv, int limit) {
+  if (!srv || limit < 1) return phpc_set_error(PHPC_ERR_ARG, "bad limit");
+  {
+    std::lock_guard<std::mutex> lk(srv->mu);
+    srv->big_limit = limit;
+  }
+  srv->cv_work.notify_all();
   return phpc_set_error(PHPC_OK, nullptr);
 }
 
