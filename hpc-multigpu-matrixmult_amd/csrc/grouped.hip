@@ -23,6 +23,13 @@ struct phpc_plan {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t graph_stream = nullptr;
+  // last launch on each stream that used the descriptors; destroy waits for these only
+  std::map<hipStream_t, hipEvent_t> last_use;
+  void mark_use(hipStream_t s) {
+    hipEvent_t &e = last_use[s];
+    if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) e = nullptr;
+    if (e) (void)hipEventRecord(e, s);
+  }
 };
 
 namespace {
@@ -219,6 +226,7 @@ extern "C" int phpc_plan_launch(phpc_plan_t p, void *stream) {
   if (!p) return phpc_set_error(PHPC_ERR_ARG, "null plan");
   int rc = launch(p, (hipStream_t)stream);
   if (rc) return rc;
+  p->mark_use((hipStream_t)stream);
   return phpc_set_error(PHPC_OK, nullptr);
 }
 
@@ -240,6 +248,7 @@ extern "C" int phpc_plan_graph_launch(phpc_plan_t p, void *stream) {
     p->graph_stream = s;
   }
   PHPC_HIP(hipGraphLaunch(p->exec, s));
+  p->mark_use(s);
   return phpc_set_error(PHPC_OK, nullptr);
 }
 
@@ -256,9 +265,11 @@ extern "C" void phpc_plan_destroy(phpc_plan_t p) {
   if (!p) return;
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
-  if (p->descs_dev) {
-    (void)hipDeviceSynchronize();
-    (void)hipFree(p->descs_dev);
+  // wait for the plan's own launches (not the whole device) before freeing what they read
+  for (auto &kv : p->last_use) {
+    (void)hipEventSynchronize(kv.second);
+    (void)hipEventDestroy(kv.second);
   }
+  if (p->descs_dev) (void)hipFree(p->descs_dev);
   delete p;
 }

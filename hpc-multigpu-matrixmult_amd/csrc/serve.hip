@@ -96,6 +96,7 @@ struct Req {
   phpc_request r;
   Kind kind;
   bool small;  // fewer output tiles than compute units: batchable
+  bool heavy;  // counts against the limit on concurrent large requests
   bool wide;   // dimensions allow the 16-byte load path
   double est;  // estimated device time (s), for shortest-first scheduling
   size_t bytes[3];
@@ -133,6 +134,7 @@ struct phpc_server {
   int pending = 0;
   int big_running = 0;  // requests that fill the GPU alone, in flight
   int big_limit = 1;    // at most this many of them at once
+  double heavy_s = 0.0;  // requests estimated shorter than this are not limited
   bool stop = false;
   std::vector<Worker *> workers;
 
@@ -153,7 +155,7 @@ std::deque<Req *>::iterator phpc_server::pick_locked() {
   auto best = queue.end();
   double best_key = 0.0;
   for (auto it = queue.begin(); it != queue.end(); ++it) {
-    if (big_running >= big_limit && !(*it)->small) continue;
+    if (big_running >= big_limit && (*it)->heavy) continue;
     if (policy == PHPC_SCHED_FIFO) return it;
     double key = (*it)->est;
     if (aging > 0.0) key -= std::chrono::duration<double>(now - (*it)->arrive).count() / aging;
@@ -300,7 +302,7 @@ void phpc_server::run(Worker *w) {
         if (it != queue.end() && admit_locked(*it)) {
           h = *it;
           queue.erase(it);
-          if (!h->small) ++big_running;
+          if (h->heavy) ++big_running;
           break;
         }
         if (stop) return;  // nothing admissible left for this worker
@@ -326,7 +328,7 @@ void phpc_server::run(Worker *w) {
     process(w, batch);
     {
       std::lock_guard<std::mutex> lk(mu);
-      if (!batch[0]->small) --big_running;
+      if (batch[0]->heavy) --big_running;
       for (Req *q : batch) {
         for (int i = 0; i < 3; ++i) in_use -= round_up(q->bytes[i]);
         release_locked(q);
@@ -417,6 +419,7 @@ extern "C" int phpc_server_submit(phpc_server_t srv, const phpc_request *req, ui
   // device-time estimate from the flop count at each path's typical rate
   const double flops = 2.0 * r.m * (double)r.n * r.k;
   q->est = 5e-5 + flops / (q->kind == kF64Plain ? 55e12 : 100e12);
+  q->heavy = !q->small && q->est >= srv->heavy_s;
   if (total > srv->arena.cap) {
     delete q;
     return phpc_set_error(PHPC_ERR_BUDGET, "request needs %zu bytes, operand budget %zu", total, srv->arena.cap);
@@ -467,11 +470,12 @@ extern "C" int phpc_server_set_policy(phpc_server_t srv, int policy, double agin
   return phpc_set_error(PHPC_OK, nullptr);
 }
 
-extern "C" int phpc_server_set_big_limit(phpc_server_t srv, int limit) {
-  if (!srv || limit < 1) return phpc_set_error(PHPC_ERR_ARG, "bad limit");
+extern "C" int phpc_server_set_big_limit(phpc_server_t srv, int limit, double min_seconds) {
+  if (!srv || limit < 1 || !(min_seconds >= 0.0)) return phpc_set_error(PHPC_ERR_ARG, "bad limit");
   {
     std::lock_guard<std::mutex> lk(srv->mu);
     srv->big_limit = limit;
+    srv->heavy_s = min_seconds;
   }
   srv->cv_work.notify_all();
   return phpc_set_error(PHPC_OK, nullptr);

@@ -471,24 +471,16 @@ __device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
   }
 }
 
-__global__ __launch_bounds__(256) void i8_level_256x256_w4(LevelArgs a) { i8_level_body<256, 256, 2, 2, 64, 3>(a); }
 __global__ __launch_bounds__(512) void i8_level_256x256_w8(LevelArgs a) { i8_level_body<256, 256, 2, 4, 64, 3>(a); }
 __global__ __launch_bounds__(256) void i8_level_128x128_w4(LevelArgs a) { i8_level_body<128, 128, 2, 2, 64, 4>(a); }
 __global__ __launch_bounds__(512) void i8_level_256x256_w8_k128(LevelArgs a) {
   i8_level_body<256, 256, 2, 4, 128, 2>(a);
 }
-__global__ __launch_bounds__(256) void i8_level_256x256_w4_k128(LevelArgs a) {
-  i8_level_body<256, 256, 2, 2, 128, 2>(a);
-}
 __global__ __launch_bounds__(512) void i8_level_256x128_w8(LevelArgs a) { i8_level_body<256, 128, 4, 2, 64, 4>(a); }
 __global__ __launch_bounds__(512) void i8_level_256x256_w8_s4(LevelArgs a) { i8_level_body<256, 256, 2, 4, 64, 4>(a); }
 __global__ __launch_bounds__(512) void i8_level_256x256_w8_s5(LevelArgs a) { i8_level_body<256, 256, 2, 4, 64, 5>(a); }
-__global__ __launch_bounds__(256) void i8_level_256x256_w4_s4(LevelArgs a) { i8_level_body<256, 256, 2, 2, 64, 4>(a); }
 __global__ __launch_bounds__(512) void i8_level_256x256_w8_k128_sd8(LevelArgs a) {
   i8_level_body<256, 256, 2, 4, 128, 2, 8>(a);
-}
-__global__ __launch_bounds__(256) void i8_level_256x256_w4_k128_sd8(LevelArgs a) {
-  i8_level_body<256, 256, 2, 2, 128, 2, 8>(a);
 }
 __global__ __launch_bounds__(512) void i8_level_256x256_w8_sd8(LevelArgs a) { i8_level_body<256, 256, 2, 4, 64, 3, 8>(a); }
 // BF16 slices on the same pipeline (FP32-accurate 3 x BF16 GEMM)
@@ -508,21 +500,17 @@ struct I8Config {
 #define I8CFG(BM_, BN_, WM_, WN_, BK_, ST_, FN_) \
   I8Config { BM_, BN_, WM_ *WN_ * 64, (size_t)(ST_)*I8Cfg<BM_, BN_, WM_, WN_, BK_, ST_>::STAGE_BYTES, FN_ }
 const I8Config kI8Configs[] = {
-    I8CFG(256, 256, 2, 2, 64, 3, i8_level_256x256_w4),        // 0: 4 waves, wave tile 128x128
-    I8CFG(256, 256, 2, 4, 64, 3, i8_level_256x256_w8),        // 1: 8 waves, wave tile 128x64
-    I8CFG(128, 128, 2, 2, 64, 4, i8_level_128x128_w4),        // 2: 4 waves, wave tile 64x64 (small problems)
-    I8CFG(256, 256, 2, 4, 128, 2, i8_level_256x256_w8_k128),  // 3: 8 waves, 128-byte slabs
-    I8CFG(256, 256, 2, 2, 128, 2, i8_level_256x256_w4_k128),  // 4: 4 waves, 128-byte slabs
-    I8CFG(256, 128, 4, 2, 64, 4, i8_level_256x128_w8),        // 5: 8 waves, wave tile 64x64
-    I8CFG(256, 256, 2, 4, 64, 4, i8_level_256x256_w8_s4),     // 6: 8 waves, 4-stage pipeline
-    I8CFG(256, 256, 2, 4, 64, 5, i8_level_256x256_w8_s5),     // 7: 8 waves, 5-stage pipeline
-    I8CFG(256, 256, 2, 2, 64, 4, i8_level_256x256_w4_s4),     // 8: 4 waves, 4-stage pipeline
-    I8CFG(256, 256, 2, 4, 128, 2, i8_level_256x256_w8_k128_sd8),  // 9: as 3, 32-bank swizzle
-    I8CFG(256, 256, 2, 2, 128, 2, i8_level_256x256_w4_k128_sd8),  // 10: as 4, 32-bank swizzle
-    I8CFG(256, 256, 2, 4, 64, 3, i8_level_256x256_w8_sd8),        // 11: as 1, 32-bank swizzle
+    I8CFG(256, 256, 2, 4, 64, 3, i8_level_256x256_w8),            // 0: 8 waves (wave tile 128x64), 3 stages
+    I8CFG(128, 128, 2, 2, 64, 4, i8_level_128x128_w4),            // 1: 4 waves, 64x64 wave tiles (small problems)
+    I8CFG(256, 256, 2, 4, 128, 2, i8_level_256x256_w8_k128),      // 2: 8 waves, 128-byte slabs (default)
+    I8CFG(256, 128, 4, 2, 64, 4, i8_level_256x128_w8),            // 3: 8 waves, wave tile 64x64
+    I8CFG(256, 256, 2, 4, 64, 4, i8_level_256x256_w8_s4),         // 4: 8 waves, 4-stage pipeline
+    I8CFG(256, 256, 2, 4, 64, 5, i8_level_256x256_w8_s5),         // 5: 8 waves, 5-stage pipeline
+    I8CFG(256, 256, 2, 4, 128, 2, i8_level_256x256_w8_k128_sd8),  // 6: as 2, 32-bank swizzle
+    I8CFG(256, 256, 2, 4, 64, 3, i8_level_256x256_w8_sd8),        // 7: as 0, 32-bank swizzle
 };
 constexpr int kNumI8 = sizeof(kI8Configs) / sizeof(kI8Configs[0]);
-int g_i8_big = 3;
+int g_i8_big = 2;  // kI8Configs index for large problems
 
 // BF16-slice level kernels (FP32-accurate 3 x BF16 GEMM), kept apart from the int8
 // table so that phpc_split_set_config can only pick int8 kernels
@@ -1083,7 +1071,7 @@ int phpc_dgemm_split_ws(const double *dA, long lda, const double *dB, long ldb, 
       if (rc) return rc;
     }
     long tiles256 = (long)((m + 255) / 256) * ((n + 255) / 256);
-    const int cfg = tiles256 >= (long)phpc_device().cus / 2 ? g_i8_big : 2;
+    const int cfg = tiles256 >= (long)phpc_device().cus / 2 ? g_i8_big : 1;
     const long piece = std::min<long>(kp, kMaxExactTermsCrt);
     int launches = 0;
     for (int t = 0; t < L; ++t) {
@@ -1162,7 +1150,7 @@ int phpc_dgemm_split_ws(const double *dA, long lda, const double *dB, long ldb, 
 
   // level GEMMs, smallest level first
   long tiles256 = (long)((m + 255) / 256) * ((n + 255) / 256);
-  const int cfg = tiles256 >= (long)phpc_device().cus / 2 ? g_i8_big : 2;
+  const int cfg = tiles256 >= (long)phpc_device().cus / 2 ? g_i8_big : 1;
   const long piece = kMaxExactTerms / kp * kp;  // whole slices per piece keep offsets aligned
   const long max_piece = piece > 0 ? piece : (kMaxExactTerms / 64) * 64;
   bool first = true;

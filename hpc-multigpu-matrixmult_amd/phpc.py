@@ -127,7 +127,7 @@ _SIGS = {
     "phpc_server_drain": (I, [vp]),
     "phpc_server_set_split_min": (I, [vp, I]),
     "phpc_server_set_policy": (I, [vp, I, ctypes.c_double]),
-    "phpc_server_set_big_limit": (I, [vp, I]),
+    "phpc_server_set_big_limit": (I, [vp, I, ctypes.c_double]),
     "phpc_server_memory": (I, [vp, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
     "phpc_server_destroy": (None, [vp]),
     # include/utils.h
@@ -440,8 +440,8 @@ class Server:
     def set_policy(self, policy, aging=0.0):
         check(lib().phpc_server_set_policy(self.h, policy, aging))
 
-    def set_big_limit(self, limit):
-        check(lib().phpc_server_set_big_limit(self.h, limit))
+    def set_big_limit(self, limit, min_seconds=0.0):
+        check(lib().phpc_server_set_big_limit(self.h, limit, min_seconds))
 
     def memory(self):
         a, b = ctypes.c_size_t(), ctypes.c_size_t()

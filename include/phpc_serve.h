@@ -70,9 +70,11 @@ int phpc_server_set_split_min(phpc_server_t server, int dim);
  * divided by `aging` so that large requests are not starved; aging = 0: no credit. */
 #define PHPC_SCHED_SJF 1
 int phpc_server_set_policy(phpc_server_t server, int policy, double aging);
-/* Requests that fill the GPU alone (output tiles >= compute units) in flight at once
- * (default 1); small requests run beside them regardless. */
-int phpc_server_set_big_limit(phpc_server_t server, int limit);
+/* Requests that fill the GPU alone (output tiles >= compute units) and whose estimated
+ * device time is at least min_seconds: at most `limit` of them in flight at once
+ * (defaults 1 and 0); every other request may start beside them. Applies to later
+ * submissions. */
+int phpc_server_set_big_limit(phpc_server_t server, int limit, double min_seconds);
 /* Operand bytes currently admitted, and the high-water mark. */
 int phpc_server_memory(phpc_server_t server, size_t *in_use, size_t *peak);
 void phpc_server_destroy(phpc_server_t server);
