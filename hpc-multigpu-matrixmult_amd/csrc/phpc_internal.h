@@ -9,6 +9,20 @@
 
 #include "phpc_gemm.h"
 
+// An event destroyed when its scope ends, error returns included.
+struct ScopedEvent {
+  hipEvent_t e = nullptr;
+  explicit ScopedEvent(unsigned flags = hipEventDefault) {
+    if (hipEventCreateWithFlags(&e, flags) != hipSuccess) e = nullptr;
+  }
+  ~ScopedEvent() {
+    if (e) (void)hipEventDestroy(e);
+  }
+  ScopedEvent(const ScopedEvent &) = delete;
+  ScopedEvent &operator=(const ScopedEvent &) = delete;
+  ScopedEvent(ScopedEvent &&o) noexcept : e(o.e) { o.e = nullptr; }
+};
+
 // ---------------------------------------------------------------------------
 // Error handling: every entry point records its status in a thread-local slot
 // (phpc_last_error) and returns it where the signature allows.

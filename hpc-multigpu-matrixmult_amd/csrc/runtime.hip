@@ -241,10 +241,9 @@ static int gemm_cuda_impl(const double *a, int lda, const double *b, int ldb, do
     if (int rc = phpc_stage_h2d(dB, (size_t)n * 8, b, (size_t)ldb * 8, (size_t)n * 8, k, s)) return rc;
   }
   if (int rc = phpc_stage_h2d(dC, (size_t)n * 8, c, (size_t)ldc * 8, (size_t)n * 8, m, s)) return rc;
-  hipEvent_t e0, e1;
-  PHPC_HIP(hipEventCreate(&e0));
-  PHPC_HIP(hipEventCreate(&e1));
-  PHPC_HIP(hipEventRecord(e0, s));
+  ScopedEvent e0, e1;
+  if (!e0.e || !e1.e) return phpc_set_error(PHPC_ERR_HIP, "event creation failed");
+  PHPC_HIP(hipEventRecord(e0.e, s));
   int rc = PHPC_OK;
   if (k > 0) {
     if (vendor) {
@@ -255,14 +254,12 @@ static int gemm_cuda_impl(const double *a, int lda, const double *b, int ldb, do
       rc = phpc_dgemm_launch(dA, k, dB, n, dC, n, m, k, n, 0, G, s);
     }
   }
-  PHPC_HIP(hipEventRecord(e1, s));
+  PHPC_HIP(hipEventRecord(e1.e, s));
   if (rc == PHPC_OK) rc = phpc_stage_d2h(c, (size_t)ldc * 8, dC, (size_t)n * 8, (size_t)n * 8, m, s);
   PHPC_HIP(hipStreamSynchronize(s));
-  float ms = 0;
-  PHPC_HIP(hipEventElapsedTime(&ms, e0, e1));
-  PHPC_HIP(hipEventDestroy(e0));
-  PHPC_HIP(hipEventDestroy(e1));
   if (rc) return rc;
+  float ms = 0;
+  PHPC_HIP(hipEventElapsedTime(&ms, e0.e, e1.e));
   if (compute_time) *compute_time = vendor ? 0.0f : ms / 1000.0f;
   return phpc_set_error(PHPC_OK, nullptr);
 }

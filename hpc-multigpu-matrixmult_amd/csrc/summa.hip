@@ -181,19 +181,16 @@ extern "C" int phpc_summa_host(int p, int q, int panel, int mode, const double *
     if (int rc = phpc_stage_h2d(dA, row, A, row, row, N, sx)) return rc;
     if (int rc = phpc_stage_h2d(dB, row, B, row, row, N, sx)) return rc;
     if (int rc = phpc_stage_h2d(dC, row, C, row, row, N, sx)) return rc;
-    hipEvent_t e0, e1;
-    PHPC_HIP(hipEventCreate(&e0));
-    PHPC_HIP(hipEventCreate(&e1));
-    PHPC_HIP(hipEventRecord(e0, sx));
+    ScopedEvent e0, e1;
+    if (!e0.e || !e1.e) return phpc_set_error(PHPC_ERR_HIP, "event creation failed");
+    PHPC_HIP(hipEventRecord(e0.e, sx));
     int rc = phpc_summa_device(p, q, panel, mode, dA, dB, dC, N, wg, sx);
-    PHPC_HIP(hipEventRecord(e1, sx));
+    PHPC_HIP(hipEventRecord(e1.e, sx));
     if (rc == PHPC_OK) rc = phpc_stage_d2h(C, row, dC, row, row, N, sx);
     PHPC_HIP(hipStreamSynchronize(sx));
-    float ms = 0;
-    PHPC_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     if (rc) return rc;
+    float ms = 0;
+    PHPC_HIP(hipEventElapsedTime(&ms, e0.e, e1.e));
     if (kernel_seconds) *kernel_seconds = ms / 1000.0f;
     return phpc_set_error(PHPC_OK, nullptr);
   }
@@ -205,11 +202,15 @@ extern "C" int phpc_summa_host(int p, int q, int panel, int mode, const double *
   // bands of whole 256-row tiles, about 16 of them
   const int hb = std::max(256, (N / 16 + 255) / 256 * 256);
   const int nb = (N + hb - 1) / hb;
-  std::vector<hipEvent_t> ev_in(nb), ev_s(nb), ev_e(nb);
+  std::vector<ScopedEvent> ev_in, ev_s, ev_e;
+  ev_in.reserve(nb);
+  ev_s.reserve(nb);
+  ev_e.reserve(nb);
   for (int i = 0; i < nb; ++i) {
-    PHPC_HIP(hipEventCreateWithFlags(&ev_in[i], hipEventDisableTiming));
-    PHPC_HIP(hipEventCreate(&ev_s[i]));
-    PHPC_HIP(hipEventCreate(&ev_e[i]));
+    ev_in.emplace_back(hipEventDisableTiming);
+    ev_s.emplace_back();
+    ev_e.emplace_back();
+    if (!ev_in.back().e || !ev_s.back().e || !ev_e.back().e) return phpc_set_error(PHPC_ERR_HIP, "event creation failed");
   }
   int rc = phpc_stage_h2d(dB, row, B, row, row, N, sx);
   for (int i = 0; i <= nb && rc == PHPC_OK; ++i) {
@@ -219,8 +220,8 @@ extern "C" int phpc_summa_host(int p, int q, int panel, int mode, const double *
       rc = phpc_stage_h2d(dA + r0 * N, row, A + r0 * N, row, row, rows, sx);
       if (!rc) rc = phpc_stage_h2d(dC + r0 * N, row, C + r0 * N, row, row, rows, sx);
       if (rc) break;
-      PHPC_HIP(hipEventRecord(ev_in[i], sx));
-      PHPC_HIP(hipStreamWaitEvent(sc, ev_in[i], 0));
+      PHPC_HIP(hipEventRecord(ev_in[i].e, sx));
+      PHPC_HIP(hipStreamWaitEvent(sc, ev_in[i].e, 0));
       GemmDesc d{};
       d.A = dA + r0 * N;
       d.B = dB;
@@ -232,27 +233,24 @@ extern "C" int phpc_summa_host(int p, int q, int panel, int mode, const double *
       d.n = N;
       d.k = N;
       d.panel = mode == PHPC_SUMMA_CHAINED ? 0 : w;
-      PHPC_HIP(hipEventRecord(ev_s[i], sc));
+      PHPC_HIP(hipEventRecord(ev_s[i].e, sc));
       rc = phpc_f64_grouped(&d, 1, G, pol.config_id, sc);
-      PHPC_HIP(hipEventRecord(ev_e[i], sc));
+      PHPC_HIP(hipEventRecord(ev_e[i].e, sc));
       if (rc) break;
     }
     if (i >= 1) {  // band i-1 back to the host once computed
       const long r0 = (long)(i - 1) * hb;
       const int rows = (int)std::min<long>(hb, N - r0);
-      PHPC_HIP(hipStreamWaitEvent(sx, ev_e[i - 1], 0));
+      PHPC_HIP(hipStreamWaitEvent(sx, ev_e[i - 1].e, 0));
       rc = phpc_stage_d2h(C + r0 * N, row, dC + r0 * N, row, row, rows, sx);
     }
   }
   (void)hipStreamSynchronize(sc);
   (void)hipStreamSynchronize(sx);
   float total = 0;
-  for (int i = 0; i < nb; ++i) {
+  for (int i = 0; i < nb && rc == PHPC_OK; ++i) {
     float ms = 0;
-    if (rc == PHPC_OK && hipEventElapsedTime(&ms, ev_s[i], ev_e[i]) == hipSuccess) total += ms;
-    (void)hipEventDestroy(ev_in[i]);
-    (void)hipEventDestroy(ev_s[i]);
-    (void)hipEventDestroy(ev_e[i]);
+    if (hipEventElapsedTime(&ms, ev_s[i].e, ev_e[i].e) == hipSuccess) total += ms;
   }
   if (rc) return rc;
   if (kernel_seconds) *kernel_seconds = total / 1000.0f;
