@@ -12,6 +12,7 @@
 // the one computing. Latency is measured from submission to completion.
 #include "phpc_internal.h"
 #include "phpc_serve.h"
+#include "arena.h"
 
 #include <algorithm>
 #include <chrono>
@@ -27,64 +28,35 @@
 namespace {
 
 using clk = std::chrono::steady_clock;
-constexpr size_t kAlign = 256;
-constexpr size_t kNone = ~(size_t)0;
+using phpc::OffsetArena;
+constexpr size_t kNone = OffsetArena::kNone;
 
-size_t round_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+size_t round_up(size_t x) { return OffsetArena::round_up(x); }
 
-// First-fit sub-allocator over one device allocation; free blocks coalesce.
-class Arena {
- public:
+// One device allocation holding every admitted request's operands.
+struct Arena {
   char *base = nullptr;
   size_t cap = 0;
+  OffsetArena offsets;
 
   int init(size_t bytes) {
-    cap = bytes / kAlign * kAlign;
+    cap = offsets.reset(bytes);
     if (cap == 0) return PHPC_ERR_BUDGET;
     if (hipMalloc(&base, cap) != hipSuccess) {
       base = nullptr;
       cap = 0;
       return PHPC_ERR_MEM;
     }
-    free_[0] = cap;
     return PHPC_OK;
   }
-  size_t alloc(size_t bytes) {
-    bytes = round_up(bytes);
-    for (auto it = free_.begin(); it != free_.end(); ++it) {
-      if (it->second < bytes) continue;
-      const size_t off = it->first, len = it->second;
-      free_.erase(it);
-      if (len > bytes) free_[off + bytes] = len - bytes;
-      return off;
-    }
-    return kNone;
-  }
-  void release(size_t off, size_t bytes) {
-    bytes = round_up(bytes);
-    auto it = free_.emplace(off, bytes).first;
-    auto nx = std::next(it);
-    if (nx != free_.end() && it->first + it->second == nx->first) {
-      it->second += nx->second;
-      free_.erase(nx);
-    }
-    if (it != free_.begin()) {
-      auto pv = std::prev(it);
-      if (pv->first + pv->second == it->first) {
-        pv->second += it->second;
-        free_.erase(it);
-      }
-    }
-  }
+  size_t alloc(size_t bytes) { return offsets.alloc(bytes); }
+  void release(size_t off, size_t bytes) { offsets.release(off, bytes); }
   void destroy() {
     if (base) (void)hipFree(base);
     base = nullptr;
     cap = 0;
-    free_.clear();
+    offsets.reset(0);
   }
-
- private:
-  std::map<size_t, size_t> free_;  // offset -> length
 };
 
 // How a request is computed. kF32 and kF64Plain requests that do not fill the GPU are
