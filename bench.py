@@ -400,7 +400,7 @@ def main():
     ap.add_argument("--no-grouped", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=1000)
-    ap.add_argument("--only", choices=["split", "fp32acc", "grouped", "sweep", "serve"],
+    ap.add_argument("--only", choices=["summa", "split", "fp32acc", "grouped", "sweep", "serve"],
                     help="run one section alone and print its JSON (for profiler runs)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy three times (counter calibration)")
@@ -425,7 +425,27 @@ def main():
 
     phpc = load_phpc()
     if args.only:
-        if args.only == "split":
+        if args.only == "summa":
+            # the headline CHAINED SUMMA at N on the --grid / --panel emulated ranks, one
+            # warm-up and `steps` timed calls
+            p, q = (int(x) for x in args.grid.split("x"))
+            A = torch.empty((args.n, args.n), dtype=torch.float64, device="cuda:0")
+            B = torch.empty_like(A)
+            C = torch.zeros_like(A)
+            phpc.fill_device(A, phpc.DIST_UNIFORM, 1)
+            phpc.fill_device(B, phpc.DIST_UNIFORM, 2)
+            sp = phpc.stream_ptr(torch.cuda.current_stream())
+            for i in range(1 + args.steps):
+                if i == 1:
+                    torch.cuda.synchronize()
+                    t0 = time.perf_counter()
+                phpc.check(phpc.lib().phpc_summa_device(p, q, args.panel, phpc.SUMMA_CHAINED, phpc.dptr(A),
+                                                         phpc.dptr(B), phpc.dptr(C), args.n, 0, sp))
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) / max(1, args.steps)
+            res = {"mode": "chained", "n": args.n, "grid": args.grid, "panel": args.panel,
+                   "tflops": round(2.0 * args.n ** 3 / dt / 1e12, 2)}
+        elif args.only == "split":
             # the split CRT GEMM at N (uniform inputs), one warm-up and steps timed calls
             A = torch.empty((args.n, args.n), dtype=torch.float64, device="cuda:0")
             B = torch.empty_like(A)

@@ -605,6 +605,181 @@ __global__ __launch_bounds__(WM *WN * 64) __attribute__((amdgpu_waves_per_eu(2))
   }
 }
 
+// ---------------------------------------------------------------------------
+// Transposed-A direct-to-LDS variant. The caller stores A transposed (At[k][m], row
+// pitch = the descriptor's lda, 16-byte aligned rows, pad columns zero), so both
+// operand slabs are k-row-major and are copied HBM -> LDS by 16-byte buffer loads
+// with no register staging and no LDS write instructions: each wave instruction fills
+// one contiguous 1 KiB piece of a k row. LDS rows are padded by 64 bytes. The MFMA
+// operand reads are the register-staged kernel's (16 consecutive doubles per lane
+// group), so the arithmetic is identical. Two stages: a slab's loads land while the
+// previous slab is computed. One panel only.
+// ---------------------------------------------------------------------------
+template <int BM, int BN, int WM, int WN>
+struct F64At {
+  static constexpr int BK = 16;
+  static constexpr int STAGES = 2;
+  static constexpr int NW = WM * WN;
+  static constexpr int WTM = BM / WM, WTN = BN / WN;
+  static constexpr int TM = WTM / 16, TN = WTN / 16;
+  static constexpr int AP = BM + 8, BP = BN + 8;  // LDS row pitches (doubles)
+  static constexpr int A_BYTES = BK * AP * 8;
+  static constexpr int STAGE_BYTES = A_BYTES + BK * BP * 8;
+  static constexpr int A_PIECES = BM / 128, B_PIECES = BN / 128;  // 1 KiB pieces per k row
+  static constexpr int A_BLOCKS = BK * A_PIECES, BLOCKS = A_BLOCKS + BK * B_PIECES;
+  static constexpr int PER = BLOCKS / NW;
+  static_assert(BM % 128 == 0 && BN % 128 == 0 && BLOCKS % NW == 0, "tile shape");
+  static_assert(STAGES * STAGE_BYTES <= 160 * 1024, "LDS budget");
+};
+
+template <int BM, int BN, int WM, int WN>
+__device__ __forceinline__ void f64_at_body(const GemmBatch &batch) {
+  using K = F64At<BM, BN, WM, WN>;
+  extern __shared__ __attribute__((aligned(1024))) char smem[];
+  const int tid = threadIdx.x;
+  const int wave = uni(tid >> 6);
+  const int lane = tid & 63;
+  const int g = lane >> 4;
+  const int li = lane & 15;
+  const int wm = wave / WN;
+  const int wn = wave % WN;
+
+  const int G = gridDim.x;
+  const int bid = blockIdx.x;
+  const bool xcd_remap = batch.xcd_remap && (G % 8) == 0;
+  const int slot = uni(xcd_remap ? (bid % 8) * (G / 8) + bid / 8 : bid);
+
+  for (int round = 0;; ++round) {
+    const int t = round * G + slot;
+    if (t >= batch.total_tiles) break;
+    const GemmDesc d = uniform_desc(batch, uni(t));
+    const int lt = t - d.tile_start;
+    const int tiles_n = d.tiles_n;
+    const int tiles_m = (d.m + BM - 1) / BM;
+    const int gm = batch.group_m;
+    const int grp = lt / (gm * tiles_n);
+    const int first_m = grp * gm;
+    const int gsz = min(tiles_m - first_m, gm);
+    const int in_grp = lt - grp * gm * tiles_n;
+    const int row0 = uni((first_m + in_grp % gsz) * BM);
+    const int col0 = uni((in_grp / gsz) * BN);
+    const int m_rem = uni(d.m - row0);
+    const int n_rem = uni(d.n - col0);
+    const long ldat = uni(d.lda), ldb = uni(d.ldb), ldc = uni(d.ldc);
+    const double *At_g = uni(d.A);
+    const double *Bg = uni(d.B) + col0;
+    gdouble *Cg = (gdouble *)(uni(d.C) + (long)row0 * ldc + col0);
+    const int Kdim = uni(d.k);
+    const int nslab = (Kdim + K::BK - 1) / K::BK;
+
+    // lane offsets of this wave's pieces within a slab (k row kr, columns c, c+1);
+    // columns past the tile's edge read zero
+    uint32_t goff[K::PER];
+    int loff[K::PER];
+#pragma unroll
+    for (int u = 0; u < K::PER; ++u) {
+      const int blk = wave + u * K::NW;
+      if (blk < K::A_BLOCKS) {
+        const int kr = blk / K::A_PIECES;
+        const int c = (blk % K::A_PIECES) * 128 + 2 * lane;
+        goff[u] = c < m_rem ? (uint32_t)(((long)kr * ldat + row0 + c) * 8) : kOOB;
+        loff[u] = kr * K::AP * 8 + (blk % K::A_PIECES) * 1024;
+      } else {
+        const int q = blk - K::A_BLOCKS;
+        const int kr = q / K::B_PIECES;
+        const int c = (q % K::B_PIECES) * 128 + 2 * lane;
+        goff[u] = c < n_rem ? (uint32_t)(((long)kr * ldb + c) * 8) : kOOB;
+        loff[u] = K::A_BYTES + kr * K::BP * 8 + (q % K::B_PIECES) * 1024;
+      }
+    }
+
+    auto issue = [&](int sidx) {
+      const int kb = sidx * K::BK;
+      const int krem = min(K::BK, Kdim - kb);
+      char *stage = smem + (sidx % K::STAGES) * K::STAGE_BYTES;
+      // k rows past the end read zero (outside the descriptors' ranges)
+      const rsrc_t rA = make_rsrc(At_g + (long)kb * ldat, (long)krem * ldat * 8);
+      const rsrc_t rB = make_rsrc(Bg + (long)kb * ldb, (long)krem * ldb * 8);
+#pragma unroll
+      for (int u = 0; u < K::PER; ++u) {
+        const int blk = wave + u * K::NW;
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(blk < K::A_BLOCKS ? rA : rB, (lds_void *)(stage + loff[u]), 16, goff[u],
+                                                  0, 0, 0);
+      }
+    };
+
+    d4 acc[K::TM][K::TN];
+#pragma unroll
+    for (int i = 0; i < K::TM; ++i)
+#pragma unroll
+      for (int j = 0; j < K::TN; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+
+    // every wave must be done reading the previous tile's stages before refilling them
+    __syncthreads();
+    if (nslab > 0) issue(0);
+    for (int sidx = 0; sidx < nslab; ++sidx) {
+      wait_vm_c<0>();
+      __syncthreads();
+      if (sidx + 1 < nslab) issue(sidx + 1);
+      const char *stage = smem + (sidx % K::STAGES) * K::STAGE_BYTES;
+      const double *At = reinterpret_cast<const double *>(stage);
+      const double *Bs = reinterpret_cast<const double *>(stage + K::A_BYTES);
+      // k beyond the last slab's rows reads as zero: fma(0, 0, c) = c
+#pragma unroll
+      for (int kk = 0; kk < K::BK / 4; ++kk) {
+        double a[K::TM], b[K::TN];
+#pragma unroll
+        for (int i = 0; i < K::TM; ++i) a[i] = At[(kk * 4 + g) * K::AP + wm * K::WTM + i * 16 + li];
+#pragma unroll
+        for (int j = 0; j < K::TN; ++j) b[j] = Bs[(kk * 4 + g) * K::BP + wn * K::WTN + j * 16 + li];
+#pragma unroll
+        for (int i = 0; i < K::TM; ++i)
+#pragma unroll
+          for (int j = 0; j < K::TN; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+      }
+    }
+
+#pragma unroll
+    for (int i = 0; i < K::TM; ++i)
+#pragma unroll
+      for (int jj = 0; jj < K::TN; ++jj) {
+        const int cc = wn * K::WTN + jj * 16 + li;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int rr = wm * K::WTM + i * 16 + 4 * r + g;
+          if (rr < m_rem && cc < n_rem) {
+            gdouble *pc = Cg + (long)rr * ldc + cc;
+            *pc = *pc + acc[i][jj][r];
+          }
+        }
+      }
+  }
+}
+
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void f64_at_256x128(GemmBatch batch) {
+  f64_at_body<256, 128, 4, 2>(batch);
+}
+
+// At[k][m] = A[m][k] for m < M, k < K; At[k][m] = 0 for M <= m < ldat (row pitch ldat)
+__global__ __launch_bounds__(256) void f64_transpose_kernel(const double *__restrict__ A, long lda, int M, int Kd,
+                                                            double *__restrict__ At, long ldat) {
+  __shared__ double tile[32][33];
+  const int k0 = blockIdx.x * 32, m0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+#pragma unroll
+  for (int r = ty; r < 32; r += 8) {
+    const int m = m0 + r, k = k0 + tx;
+    tile[r][tx] = (m < M && k < Kd) ? A[(long)m * lda + k] : 0.0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = ty; r < 32; r += 8) {
+    const int k = k0 + r, m = m0 + tx;
+    if (k < Kd && m < ldat) At[(long)k * ldat + m] = tile[tx][r];
+  }
+}
+
 typedef void (*KernelFn)(GemmBatch);
 
 // One tile configuration: kernels for [panel grouping][16-byte loads]. Direct-to-LDS
@@ -616,6 +791,7 @@ struct ConfigEntry {
   size_t lds_bytes[2];  // dynamic LDS for [vec1][vec2] kernels
   bool panels;          // panel-grouped variants fit without spills
   KernelFn fn[2][2];
+  bool trans_a = false;  // the 16-byte kernel reads A transposed (descriptor A = At, lda = its pitch)
 };
 
 #define PHPC_V1(BM_, BN_, WM_, WN_, BK_, MP_, VEC_) f64_gemm_kernel<BM_, BN_, WM_, WN_, BK_, MP_, VEC_>
@@ -665,6 +841,13 @@ const ConfigEntry kConfigs[] = {
     {128, 128, 256, 16, {0, LDS_BYTES(128, 128, 2)}, false,
      {{PHPC_V1(128, 128, 2, 2, 16, false, 1), PHPC_LDS(128, 128, 2, 2, 2, false)},
       {PHPC_V1(128, 128, 2, 2, 16, false, 1), PHPC_LDS(128, 128, 2, 2, 2, false)}}},
+    // 10: transposed-A direct-to-LDS, 8 waves (wave tile 64x64), 256x128 tiles, 2 stages;
+    //     single panel only. The 8-byte slots are config 6's kernels: when B rules out
+    //     16-byte loads the launcher runs config 6 on the untransposed A instead.
+    {256, 128, 512, 16, {0, 2 * (size_t)F64At<256, 128, 4, 2>::STAGE_BYTES}, false,
+     {{PHPC_V1(256, 128, 4, 2, 16, false, 1), f64_at_256x128},
+      {PHPC_V1(256, 128, 4, 2, 16, false, 1), f64_at_256x128}},
+     true},
 };
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 
@@ -748,12 +931,23 @@ int phpc_f64_pick_config(const GemmDesc *descs, int ndesc) {
   return 5;
 }
 
-int phpc_f64_grouped_dev(const GemmDesc *descs_dev, const GemmDesc *descs_host, int ndesc, int grid,
-                         int config_id, hipStream_t stream) {
+namespace {
+
+// Configuration for large single-panel work reached through host descriptors
+// (phpc_f64_grouped), which can stage a transposed copy of A first.
+int g_big_single_host = 10;
+
+bool b_vec2_ok(const GemmDesc &d) { return aligned16(d.B) && (d.ldb % 2 == 0) && (d.n % 2 == 0); }
+
+int launch_grouped(const GemmDesc *descs_dev, const GemmDesc *descs_host, int ndesc, int grid, int config_id,
+                   hipStream_t stream, bool a_transposed) {
   if (ndesc <= 0) return PHPC_OK;
   if (config_id < 0 || config_id >= kNumConfigs)
     return phpc_set_error(PHPC_ERR_ARG, "bad FP64 config id %d", config_id);
   const ConfigEntry &ce = kConfigs[config_id];
+  if (ce.trans_a != a_transposed)
+    return phpc_set_error(PHPC_ERR_ARG, "FP64 config %d %s a transposed A", config_id,
+                          ce.trans_a ? "needs" : "does not take");
   bool mp = false, vec2 = true;
   int total = 0;
   for (int i = 0; i < ndesc; ++i) {
@@ -762,13 +956,14 @@ int phpc_f64_grouped_dev(const GemmDesc *descs_dev, const GemmDesc *descs_host, 
     // 32-bit buffer offsets: one A tile (bm rows) and one B slab (bk rows) must span < 2 GiB
     if ((long)ce.bm * d.lda * 8 >= 0x7fffffffL || (long)ce.bk * d.ldb * 8 >= 0x7fffffffL)
       return phpc_set_error(PHPC_ERR_ARG, "leading dimension too large (lda=%ld, ldb=%ld)", d.lda, d.ldb);
-    // 16-byte loads need 16-byte aligned rows and even k / n boundaries
-    const bool a_ok = aligned16(d.A) && (d.lda % 2 == 0) && (d.k % 2 == 0) &&
-                      (d.panel <= 0 || d.panel >= d.k || d.panel % 2 == 0);
-    const bool b_ok = aligned16(d.B) && (d.ldb % 2 == 0) && (d.n % 2 == 0);
-    if (!a_ok || !b_ok) vec2 = false;
+    // 16-byte loads need 16-byte aligned rows and even k / n boundaries (a transposed A
+    // is stored that way by construction)
+    const bool a_ok = a_transposed || (aligned16(d.A) && (d.lda % 2 == 0) && (d.k % 2 == 0) &&
+                                       (d.panel <= 0 || d.panel >= d.k || d.panel % 2 == 0));
+    if (!a_ok || !b_vec2_ok(d)) vec2 = false;
     total = d.tile_start + ((d.m + ce.bm - 1) / ce.bm) * d.tiles_n;
   }
+  if (a_transposed && !vec2) return phpc_set_error(PHPC_ERR_ARG, "transposed-A kernels need 16-byte B rows");
   if (mp && !ce.panels)
     return phpc_set_error(PHPC_ERR_ARG, "FP64 config %d does not support panel grouping", config_id);
   if (total <= 0) return PHPC_OK;
@@ -791,6 +986,63 @@ int phpc_f64_grouped_dev(const GemmDesc *descs_dev, const GemmDesc *descs_host, 
   return PHPC_OK;
 }
 
+// Transposed copies of A for the transposed-A configurations: one per distinct
+// (A, lda, m, k), in a process-wide workspace used in stream order.
+SharedWorkspace g_at_ws;
+
+int launch_live(std::vector<GemmDesc> &live, int grid, int config_id, hipStream_t stream, bool a_transposed);
+
+int launch_transposed(std::vector<GemmDesc> &live, int grid, int config_id, hipStream_t stream) {
+  struct Src {
+    const double *A;
+    long lda;
+    int m, k;
+    long ldat;
+    size_t off;  // doubles
+  };
+  std::vector<Src> srcs;
+  size_t total = 0;
+  std::vector<int> which(live.size());
+  for (size_t i = 0; i < live.size(); ++i) {
+    const GemmDesc &d = live[i];
+    int hit = -1;
+    for (size_t j = 0; j < srcs.size(); ++j)
+      if (srcs[j].A == d.A && srcs[j].lda == d.lda && srcs[j].m == d.m && srcs[j].k == d.k) hit = (int)j;
+    if (hit < 0) {
+      const long ldat = ((long)d.m + 1) / 2 * 2;
+      srcs.push_back({d.A, d.lda, d.m, d.k, ldat, total});
+      total += ((size_t)d.k * ldat + 31) / 32 * 32;  // 256-byte aligned copies
+      hit = (int)srcs.size() - 1;
+    }
+    which[i] = hit;
+  }
+  std::lock_guard<std::mutex> lk(g_at_ws.mu);
+  g_at_ws.after_previous(stream);
+  double *ws = static_cast<double *>(g_at_ws.ws.get(total * sizeof(double), 0, stream));
+  if (!ws) return phpc_set_error(PHPC_ERR_MEM, "transposed-A workspace allocation failed (%zu bytes)", total * 8);
+  for (const Src &c : srcs) {
+    const dim3 tg((unsigned)((c.k + 31) / 32), (unsigned)((c.ldat + 31) / 32));
+    hipLaunchKernelGGL(f64_transpose_kernel, tg, dim3(256), 0, stream, c.A, c.lda, c.m, c.k, ws + c.off, c.ldat);
+    PHPC_HIP(hipGetLastError());
+  }
+  for (size_t i = 0; i < live.size(); ++i) {
+    live[i].A = ws + srcs[which[i]].off;
+    live[i].lda = srcs[which[i]].ldat;
+  }
+  const int rc = launch_live(live, grid, config_id, stream, true);
+  g_at_ws.mark_used(stream);
+  return rc;
+}
+
+}  // namespace
+
+int phpc_f64_grouped_dev(const GemmDesc *descs_dev, const GemmDesc *descs_host, int ndesc, int grid,
+                         int config_id, hipStream_t stream) {
+  return launch_grouped(descs_dev, descs_host, ndesc, grid, config_id, stream, false);
+}
+
+bool phpc_f64_config_trans_a(int id) { return id >= 0 && id < kNumConfigs && kConfigs[id].trans_a; }
+
 int phpc_f64_grouped(const GemmDesc *descs_in, int ndesc, int grid, int config_id,
                      hipStream_t stream) {
   if (ndesc <= 0) return PHPC_OK;
@@ -799,7 +1051,22 @@ int phpc_f64_grouped(const GemmDesc *descs_in, int ndesc, int grid, int config_i
     if (d.m < 0 || d.n < 0 || d.k < 0) return phpc_set_error(PHPC_ERR_ARG, "negative GEMM size");
   }
   std::vector<GemmDesc> descs(descs_in, descs_in + ndesc);
-  if (config_id < 0) config_id = phpc_f64_pick_config(descs.data(), ndesc);
+  if (config_id < 0) {
+    config_id = phpc_f64_pick_config(descs.data(), ndesc);
+    if (config_id == g_big_single) config_id = g_big_single_host;
+  }
+  if (config_id < 0 || config_id >= kNumConfigs)
+    return phpc_set_error(PHPC_ERR_ARG, "bad FP64 config id %d", config_id);
+  if (kConfigs[config_id].trans_a) {
+    // the transposed copy needs 16-byte B rows, one panel and a grid-sized m; otherwise
+    // the same tile runs register-staged on A as it is
+    bool ok = true;
+    for (const GemmDesc &d : descs)
+      if (d.m > 0 && d.n > 0 && d.k > 0 &&
+          (!b_vec2_ok(d) || (d.panel > 0 && d.panel < d.k) || ((long)d.m + 1) / 2 * 2 / 32 >= 65535))
+        ok = false;
+    if (!ok) config_id = 6;
+  }
   const ConfigEntry &ce = kConfigs[config_id];
   // drop empty GEMMs; k == 0 leaves C unchanged (C += 0 adds +0.0 only to tiles we skip)
   std::vector<GemmDesc> live;
@@ -813,7 +1080,15 @@ int phpc_f64_grouped(const GemmDesc *descs_in, int ndesc, int grid, int config_i
     live.push_back(d);
   }
   if (live.empty()) return PHPC_OK;
-  if (live.size() == 1) return phpc_f64_grouped_dev(nullptr, live.data(), 1, grid, config_id, stream);
+  if (ce.trans_a) return launch_transposed(live, grid, config_id, stream);
+  return launch_live(live, grid, config_id, stream, false);
+}
+
+namespace {
+
+// One grouped launch of prepared descriptors (host copies in `live`).
+int launch_live(std::vector<GemmDesc> &live, int grid, int config_id, hipStream_t stream, bool a_transposed) {
+  if (live.size() == 1) return launch_grouped(nullptr, live.data(), 1, grid, config_id, stream, a_transposed);
   // Several GEMMs: descriptors go through one process-wide device buffer. Callers on
   // other streams may still have a launch queued that reads it, so the buffer is
   // rewritten only after the previous user's launch has finished (g_desc_done).
@@ -839,18 +1114,22 @@ int phpc_f64_grouped(const GemmDesc *descs_in, int ndesc, int grid, int config_i
   if (!g_desc_done) PHPC_HIP(hipEventCreateWithFlags(&g_desc_done, hipEventDisableTiming));
   memcpy(g_desc_pinned, live.data(), bytes);
   PHPC_HIP(hipMemcpyAsync(g_desc_dev, g_desc_pinned, bytes, hipMemcpyHostToDevice, stream));
-  const int rc = phpc_f64_grouped_dev(g_desc_dev, live.data(), (int)live.size(), grid, config_id, stream);
+  const int rc = launch_grouped(g_desc_dev, live.data(), (int)live.size(), grid, config_id, stream, a_transposed);
   PHPC_HIP(hipEventRecord(g_desc_done, stream));
   return rc;
 }
 
+}  // namespace
+
 // Tuning knobs for the autotuner: 0 = config for large panel-grouped problems, 1 =
 // tile-row swizzle width, 2 = XCD-contiguous tile order on/off, 3 = config for large
-// single-panel problems.
+// single-panel problems on device descriptors, 4 = the same through host descriptors
+// (may be a transposed-A configuration).
 extern "C" int phpc_set_tuning(int key, int value) {
   switch (key) {
     case 0:
-      if (value < 0 || value >= kNumConfigs) return phpc_set_error(PHPC_ERR_ARG, "config %d out of range", value);
+      if (value < 0 || value >= kNumConfigs || kConfigs[value].trans_a)
+        return phpc_set_error(PHPC_ERR_ARG, "config %d out of range", value);
       g_big_config = value;
       break;
     case 1:
@@ -861,8 +1140,13 @@ extern "C" int phpc_set_tuning(int key, int value) {
       g_xcd_remap = value ? 1 : 0;
       break;
     case 3:
-      if (value < 0 || value >= kNumConfigs) return phpc_set_error(PHPC_ERR_ARG, "config %d out of range", value);
+      if (value < 0 || value >= kNumConfigs || kConfigs[value].trans_a)
+        return phpc_set_error(PHPC_ERR_ARG, "config %d out of range", value);
       g_big_single = value;
+      break;
+    case 4:
+      if (value < 0 || value >= kNumConfigs) return phpc_set_error(PHPC_ERR_ARG, "config %d out of range", value);
+      g_big_single_host = value;
       break;
     default:
       return phpc_set_error(PHPC_ERR_ARG, "unknown tuning key %d", key);

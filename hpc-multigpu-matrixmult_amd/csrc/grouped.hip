@@ -183,6 +183,7 @@ extern "C" int phpc_plan_create(int dtype, const phpc_gemm_item *items, int coun
       (void)hipEventCreate(&e1);
       for (int c = 0; c < ncfg; ++c) {
         if (dtype == PHPC_F64 && mp && !phpc_f64_config_panels(c)) continue;
+        if (dtype == PHPC_F64 && phpc_f64_config_trans_a(c)) continue;
         if (dtype == PHPC_F32X3 && c != 2) continue;
         if (plan_upload(p, c, s) != PHPC_OK) continue;
         p->grid = 0;

@@ -113,6 +113,9 @@ int phpc_f64_grouped_dev(const GemmDesc *descs_dev, const GemmDesc *descs_host, 
 int phpc_f64_num_configs();
 F64Config phpc_f64_config(int id);
 int phpc_f64_pick_config(const GemmDesc *descs, int ndesc);
+// Configurations whose kernels read a transposed copy of A, which only phpc_f64_grouped
+// (host descriptors) stages; phpc_f64_grouped_dev rejects them.
+bool phpc_f64_config_trans_a(int id);
 
 // Strided block pack: dst[r*ldd + c] = src[r*lds + c] for r < rows, c < cols.
 // Host <-> device copies through pinned staging buffers (rows x width bytes, strided).

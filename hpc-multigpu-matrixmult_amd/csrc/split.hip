@@ -220,6 +220,11 @@ struct LevelArgs {
   // BF16 mode (operands are BF16 slices): C32[i,j] += the FP32 sum
   float *C32;
   long ldc32;
+  // BF16 mode: the k range may be up to three segments of one sum. Segment s starts at
+  // byte seg[s] of the kl range (seg[0] = 0; seg1 = 0: one segment) and reads A from
+  // row byte (k - seg[s]) and B from row byte (k - seg[s]) + segb[s].
+  int seg1, seg2;
+  int segb1, segb2;
 };
 
 // x mod m in the symmetric range (m odd: |r| <= (m-1)/2; m = 256: [-128, 127]) for
@@ -293,6 +298,8 @@ __device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
   const int8_t *A8 = uni(args.A8) + uni(args.a0);
   const int8_t *B8 = uni(args.B8) + uni(args.b0);
   const int nslab = kl / K::BK;
+  const int sg1 = BF ? uni(args.seg1) : 0, sg2 = BF ? uni(args.seg2) : 0;
+  const int sb1 = BF ? uni(args.segb1) : 0, sb2 = BF ? uni(args.segb2) : 0;
 
   // LDS operand offsets per k step: lane reads 16 bytes of row (l % 32), chunk
   // 2*kk + l/32, at its swizzled slot
@@ -343,13 +350,27 @@ __device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
 
     auto issue = [&](int sidx) {
       const int kb = sidx * K::BK;
+      int ka = kb, kbb = kb;
+      if constexpr (BF) {
+        // segments start on slab boundaries, so a slab lies in one segment
+        if (sg1 > 0 && kb >= sg1) {
+          if (sg2 > 0 && kb >= sg2) {
+            ka = kb - sg2;
+            kbb = ka + sb2;
+          } else {
+            ka = kb - sg1;
+            kbb = ka + sb1;
+          }
+        }
+      }
       char *stage = smem + (sidx % STAGES) * K::STAGE_BYTES;
 #pragma unroll
       for (int u = 0; u < K::PER; ++u) {
         const int blk = wave + u * K::NW;
-        // the range check applies to the lane offset (kOOB reads zero); kb rides in soffset
-        __builtin_amdgcn_raw_ptr_buffer_load_lds(blk < K::A_BLOCKS ? rA : rB, (lds_void *)(stage + blk * 1024), 16,
-                                                  goff[u], kb, 0, 0);
+        const bool isA = blk < K::A_BLOCKS;
+        // the range check applies to the lane offset (kOOB reads zero); k rides in soffset
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(isA ? rA : rB, (lds_void *)(stage + blk * 1024), 16, goff[u],
+                                                  isA ? ka : kbb, 0, 0);
       }
     };
 
@@ -1197,9 +1218,10 @@ int phpc_dgemm_split_ws(const double *dA, long lda, const double *dB, long ldb, 
 
 // FP32-accurate GEMM from BF16 slice planes: C += A*B (FP32, row-major, device pointers).
 // A3 rows hold [hi | mid | lo] and B3 rows (columns of B) [lo | mid | hi], so the six
-// products with slice-index sum <= 2 form three contiguous k ranges, run smallest first:
-// A3[:, 0:3kp] . B3[:, 0:3kp] = hi.lo + mid.mid + lo.hi, A3[:, 0:2kp] . B3[:, kp:3kp] =
-// hi.mid + mid.hi, A3[:, 0:kp] . B3[:, 2kp:3kp] = hi.hi, each adding its FP32 sums to C.
+// products with slice-index sum <= 2 form three contiguous k ranges, summed smallest first
+// into one FP32 accumulator by one launch (three k segments): A3[:, 0:3kp] . B3[:, 0:3kp]
+// = hi.lo + mid.mid + lo.hi, then A3[:, 0:2kp] . B3[:, kp:3kp] = hi.mid + mid.hi, then
+// A3[:, 0:kp] . B3[:, 2kp:3kp] = hi.hi; the sum is added to C once.
 int phpc_sgemm_bf16x3_planes(const float *dA, long lda, const float *dB, long ldb, float *dC, long ldc, int m, int k,
                              int n, hipStream_t s, Workspace *ws) {
   const long kp = ((long)k + 63) / 64 * 64;
@@ -1218,20 +1240,20 @@ int phpc_sgemm_bf16x3_planes(const float *dA, long lda, const float *dB, long ld
   }
   const long tiles256 = (long)((m + 255) / 256) * ((n + 255) / 256);
   const int cfg = tiles256 >= (long)phpc_device().cus / 2 ? 0 : 1;
-  for (int lev = 3; lev >= 1; --lev) {
-    LevelArgs a{};
-    a.A8 = (const int8_t *)A3;
-    a.B8 = (const int8_t *)B3;
-    a.lda8 = 3 * kp * 2;
-    a.ldb8 = 3 * kp * 2;
-    a.a0 = 0;
-    a.b0 = (3 - lev) * kp * 2;
-    a.m = m;
-    a.n = n;
-    a.kl = (int)(lev * kp * 2);
-    a.C32 = dC;
-    a.ldc32 = ldc;
-    if (int rc = launch_bf_level(a, cfg, s)) return rc;
-  }
-  return PHPC_OK;
+  const int kp2 = (int)(kp * 2);  // bytes of one plane row
+  LevelArgs a{};
+  a.A8 = (const int8_t *)A3;
+  a.B8 = (const int8_t *)B3;
+  a.lda8 = 3L * kp2;
+  a.ldb8 = 3L * kp2;
+  a.m = m;
+  a.n = n;
+  a.kl = 6 * kp2;
+  a.seg1 = 3 * kp2;
+  a.seg2 = 5 * kp2;
+  a.segb1 = kp2;
+  a.segb2 = 2 * kp2;
+  a.C32 = dC;
+  a.ldc32 = ldc;
+  return launch_bf_level(a, cfg, s);
 }
