@@ -608,8 +608,9 @@ def main():
             "N": N,
             "grid": f"{p}x{q}",
             "panel": panel,
-            "path": "plain FP64 MFMA, SUMMA CHAINED schedule: one persistent launch per pass, "
-                    "256x128 tiles, one FMA chain per output over the 64 panels",
+            "path": "plain FP64 MFMA, SUMMA CHAINED schedule: A row blocks transposed once per pass, "
+                    "then one persistent launch, 128x128 tiles fed HBM->LDS directly, one FMA chain per "
+                    "output over the 64 panels",
             "flops_per_step": flops,
         },
         "accuracy": {"err_cw_max_sampled": err_cw, "err_nw_sampled": err_nw, "samples": 4096,
@@ -621,8 +622,9 @@ def main():
             "unit": "TFLOP/s",
             "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
             "traffic": traffic_from_profiles(workload),
-            "kernel": "f64_gemm_kernel<256,128,4,2,BK=16,MP=0,VEC=2>",
+            "kernel": "f64_at_128x128 (transposed-A, direct-to-LDS, 2 workgroups per CU)",
             "kernel_ms_avg": round(kernel_ms, 3),
+            "timed_unit": "one SUMMA pass: 4 A row-block transposes (~1% of the pass) + the GEMM launch",
         },
         "wall_s_timed": round(wall, 3),
         "summa_panel_exact_tflops": round(flops / (exact_ms * 1e-3) / 1e12, 3),

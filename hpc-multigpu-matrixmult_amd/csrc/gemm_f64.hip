@@ -693,13 +693,19 @@ __device__ __forceinline__ void f64_at_body(const GemmBatch &batch) {
       }
     }
 
+    // slab bases advance by BK rows; a slab's byte range is BK rows (fewer for the last
+    // slab, whose missing k rows then read zero). The launcher keeps BK rows < 2 GiB.
+    const int full_a = (int)(K::BK * ldat * 8), full_b = (int)(K::BK * ldb * 8);
     auto issue = [&](int sidx) {
       const int kb = sidx * K::BK;
       const int krem = min(K::BK, Kdim - kb);
       char *stage = smem + (sidx % K::STAGES) * K::STAGE_BYTES;
-      // k rows past the end read zero (outside the descriptors' ranges)
-      const rsrc_t rA = make_rsrc(At_g + (long)kb * ldat, (long)krem * ldat * 8);
-      const rsrc_t rB = make_rsrc(Bg + (long)kb * ldb, (long)krem * ldb * 8);
+      const int na = krem == K::BK ? full_a : krem * (int)ldat * 8;
+      const int nb = krem == K::BK ? full_b : krem * (int)ldb * 8;
+      const rsrc_t rA = __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(At_g + (long)kb * ldat), (short)0, na,
+                                                          0x00020000);
+      const rsrc_t rB =
+          __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(Bg + (long)kb * ldb), (short)0, nb, 0x00020000);
 #pragma unroll
       for (int u = 0; u < K::PER; ++u) {
         const int blk = wave + u * K::NW;
@@ -720,13 +726,11 @@ __device__ __forceinline__ void f64_at_body(const GemmBatch &batch) {
     for (int sidx = 0; sidx < nslab; ++sidx) {
       wait_vm_c<0>();
       __syncthreads();
-      if (sidx + 1 < nslab) issue(sidx + 1);
       const char *stage = smem + (sidx % K::STAGES) * K::STAGE_BYTES;
       const double *At = reinterpret_cast<const double *>(stage);
       const double *Bs = reinterpret_cast<const double *>(stage + K::A_BYTES);
       // k beyond the last slab's rows reads as zero: fma(0, 0, c) = c
-#pragma unroll
-      for (int kk = 0; kk < K::BK / 4; ++kk) {
+      auto kstep = [&](int kk) {
         double a[K::TM], b[K::TN];
 #pragma unroll
         for (int i = 0; i < K::TM; ++i) a[i] = At[(kk * 4 + g) * K::AP + wm * K::WTM + i * 16 + li];
@@ -737,7 +741,13 @@ __device__ __forceinline__ void f64_at_body(const GemmBatch &batch) {
 #pragma unroll
           for (int j = 0; j < K::TN; ++j)
             acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
-      }
+      };
+      // the first k step's LDS reads go out right after the barrier; the next slab's
+      // loads are issued behind its MFMAs
+      kstep(0);
+      if (sidx + 1 < nslab) issue(sidx + 1);
+#pragma unroll
+      for (int kk = 1; kk < K::BK / 4; ++kk) kstep(kk);
     }
 
 #pragma unroll
@@ -759,6 +769,9 @@ __device__ __forceinline__ void f64_at_body(const GemmBatch &batch) {
 
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void f64_at_256x128(GemmBatch batch) {
   f64_at_body<256, 128, 4, 2>(batch);
+}
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void f64_at_128x128(GemmBatch batch) {
+  f64_at_body<128, 128, 2, 2>(batch);
 }
 
 // At[k][m] = A[m][k] for m < M, k < K; At[k][m] = 0 for M <= m < ldat (row pitch ldat)
@@ -848,6 +861,12 @@ const ConfigEntry kConfigs[] = {
      {{PHPC_V1(256, 128, 4, 2, 16, false, 1), f64_at_256x128},
       {PHPC_V1(256, 128, 4, 2, 16, false, 1), f64_at_256x128}},
      true},
+    // 11: transposed-A direct-to-LDS, 4 waves (wave tile 64x64), 128x128 tiles, 2 stages
+    //     (68 KiB, two workgroups per CU); single panel only. 8-byte slots: config 8's kernel.
+    {128, 128, 256, 16, {0, 2 * (size_t)F64At<128, 128, 2, 2>::STAGE_BYTES}, false,
+     {{PHPC_V1(128, 128, 2, 2, 16, false, 1), f64_at_128x128},
+      {PHPC_V1(128, 128, 2, 2, 16, false, 1), f64_at_128x128}},
+     true},
 };
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 
@@ -935,7 +954,7 @@ namespace {
 
 // Configuration for large single-panel work reached through host descriptors
 // (phpc_f64_grouped), which can stage a transposed copy of A first.
-int g_big_single_host = 10;
+int g_big_single_host = 11;
 
 bool b_vec2_ok(const GemmDesc &d) { return aligned16(d.B) && (d.ldb % 2 == 0) && (d.n % 2 == 0); }
 
@@ -1065,7 +1084,7 @@ int phpc_f64_grouped(const GemmDesc *descs_in, int ndesc, int grid, int config_i
       if (d.m > 0 && d.n > 0 && d.k > 0 &&
           (!b_vec2_ok(d) || (d.panel > 0 && d.panel < d.k) || ((long)d.m + 1) / 2 * 2 / 32 >= 65535))
         ok = false;
-    if (!ok) config_id = 6;
+    if (!ok) config_id = kConfigs[config_id].bm == 256 ? 6 : 8;
   }
   const ConfigEntry &ce = kConfigs[config_id];
   // drop empty GEMMs; k == 0 leaves C unchanged (C += 0 adds +0.0 only to tiles we skip)
