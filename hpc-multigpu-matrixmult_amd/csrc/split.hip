@@ -348,9 +348,11 @@ __device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
       goff[u] = r < rows ? (uint32_t)((long)r * ld + c * 16) : kOOB;
     }
 
-    auto issue = [&](int sidx) {
+    // k byte offsets of slab sidx in A's and B's rows
+    auto slab_k = [&](int sidx, int &ka, int &kbb) {
       const int kb = sidx * K::BK;
-      int ka = kb, kbb = kb;
+      ka = kb;
+      kbb = kb;
       if constexpr (BF) {
         // segments start on slab boundaries, so a slab lies in one segment
         if (sg1 > 0 && kb >= sg1) {
@@ -363,6 +365,10 @@ __device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
           }
         }
       }
+    };
+    auto issue = [&](int sidx) {
+      int ka, kbb;
+      slab_k(sidx, ka, kbb);
       char *stage = smem + (sidx % STAGES) * K::STAGE_BYTES;
 #pragma unroll
       for (int u = 0; u < K::PER; ++u) {
@@ -398,6 +404,7 @@ __device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
         wait_vm_c<3 * K::PER>();
       __syncthreads();
       if (sidx + STAGES - 1 < nslab) issue(sidx + STAGES - 1);
+
       const char *stage = smem + (sidx % STAGES) * K::STAGE_BYTES;
 #pragma unroll
       for (int kk = 0; kk < K::KSTEPS; ++kk) {
@@ -681,59 +688,67 @@ double split_error_estimate(int s, double ratio_a, double ratio_b, int k) {
 // C += 2^(e_i + f_j - 2p) C'. The moduli product M must exceed 2 max|C'|, i.e.
 // sum log2 m_t >= 2p + ceil(log2 k) + 2.
 constexpr int kMaxMods = 20;
-const int kModuli[kMaxMods] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227,
-                               223, 217, 211, 199, 197, 193, 191, 181, 179, 173};
+constexpr int kModuli[kMaxMods] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227,
+                                   223, 217, 211, 199, 197, 193, 191, 181, 179, 173};
 
+// Garner's recurrence for digit t: v_t = r_t inv_t + sum_{s<t} v_s q[s][t] (mod m_t),
+// inv_t = (m_1 ... m_{t-1})^-1 and q[s][t] = -(m_1 ... m_{s-1}) inv_t mod m_t. Both are
+// stored in the symmetric range, which fits int8 for every modulus <= 256, and q is
+// packed four to a word by s (qp[t][w] byte b = q[4w+b][t], zero for 4w+b >= t) so the
+// sum runs as int8 dot products against the packed digits.
 struct CrtConsts {
   int L;
   int m[kMaxMods];
-  int inv[kMaxMods];             // (m_1 ... m_{t-1})^-1 mod m_t
-  int qinv[kMaxMods][kMaxMods];  // qinv[s][t] = -(m_1 ... m_{s-1}) inv_t mod m_t, in [0, m_t), s < t
-  float rmf[kMaxMods];           // 1 / m, rounded to FP32
-  float c12[kMaxMods][6];        // 2^(12 i) mod m
+  int invs[kMaxMods];   // inv_t, symmetric
+  int qp[kMaxMods][5];  // packed symmetric q[s][t]
+  float rmf[kMaxMods];  // 1 / m, rounded to FP32
+  uint32_t c8[kMaxMods][3];  // byte i of word w: 2^(8 (4w + i)) mod m, for the 9 bytes of |x| < 2^72
 };
 static_assert(sizeof(CrtConsts) <= 2560, "CRT constants travel as a kernel argument");
 
-// Symmetric residue of an integer-valued float 0 <= s < 2^23 (or |s| < 2^23 with the
-// correction steps below) modulo m <= 256: the quotient estimate floor(s / m) from the
-// rounded reciprocal is off by at most one (|s/m| 2^-22 < 2^-1), and q*m and s - q*m are
-// exact in FP32 (below 2^24).
-__device__ __forceinline__ int smod_f(float s, int m, float rmf) {
-  const float mf = (float)m;
-  const float q = floorf(s * rmf);
-  float r = fmaf(-q, mf, s);
-  if (r < 0.0f) r += mf;
-  if (r >= mf) r -= mf;
-  int ri = (int)r;
-  if (ri > (m - 1) / 2) ri -= m;
-  return ri;
+// Lowest digit index S whose Horner prefix (digits S..L-1, |value| <= prod m_S..m_{L-1} / 2)
+// stays below 2^53: those steps are exact in plain FP64.
+constexpr int crt_plain_start(int L) {
+  double prod = kModuli[L - 1];
+  int s = L - 1;
+  while (s > 0 && prod * kModuli[s - 1] <= 9007199254740992.0) {
+    prod *= kModuli[s - 1];
+    --s;
+  }
+  return s;
 }
 
 // Symmetric residues of one integer-valued double x (|x| < 2^72) for the first L
-// moduli. |x| is cut into six 12-bit chunks; sum_i chunk_i (2^(12 i) mod m) < 6 * 2^20
-// is exact in FP32, and so is its reduction mod m.
+// moduli. |x| is cut into nine bytes (three words); for each odd modulus the sum
+// sum_i byte_i (2^(8 i) mod m) <= 9 * 255^2 < 2^20 is two int8 dot products, exact, and
+// its symmetric residue is sum - m rint(sum / m): |sum / m| < 3400 keeps the FP32
+// quotient's rounding (< 2^-11) below the 1 / (2m) distance of sum / m from a
+// half-integer, so rint is exact. m_0 = 256 takes the low byte.
 template <int L>
 __device__ __forceinline__ void residues_of(double x, const CrtConsts &cc, int (&r)[L]) {
   const double ax = fabs(x);
   const double h = trunc(ldexp(ax, -48));              // < 2^24
   const uint64_t lo = (uint64_t)(ax - ldexp(h, 48));   // exact, < 2^48
   const uint32_t hi = (uint32_t)h;
-  float ch[6];
-  ch[0] = (float)(uint32_t)(lo & 0xFFF);
-  ch[1] = (float)(uint32_t)((lo >> 12) & 0xFFF);
-  ch[2] = (float)(uint32_t)((lo >> 24) & 0xFFF);
-  ch[3] = (float)(uint32_t)((lo >> 36) & 0xFFF);
-  ch[4] = (float)(hi & 0xFFF);
-  ch[5] = (float)(hi >> 12);
+  const uint32_t w0 = (uint32_t)lo;                          // bytes 0-3
+  const uint32_t w1 = (uint32_t)(lo >> 32) | (hi << 16);     // bytes 4-5 of lo, 0-1 of hi
+  const uint32_t w2 = hi >> 16;                              // byte 2 of hi
+  const bool neg = x < 0.0;
+  {
+    int v = (int)(int8_t)(w0 & 0xffu);
+    if (neg) v = -v;
+    if (v == 128) v = -128;  // keep [-128, 127]
+    r[0] = v;
+  }
 #pragma unroll
-  for (int t = 0; t < L; ++t) {
-    float sum = ch[0];
-#pragma unroll
-    for (int i = 1; i < 6; ++i) sum = fmaf(ch[i], cc.c12[t][i], sum);
-    int v = smod_f(sum, cc.m[t], cc.rmf[t]);
-    if (x < 0.0) v = -v;
-    if (v == 128) v = -128;  // m = 256: keep [-128, 127]
-    r[t] = v;
+  for (int t = 1; t < L; ++t) {
+    uint32_t sum = __builtin_amdgcn_udot4(w0, cc.c8[t][0], 0u, false);
+    sum = __builtin_amdgcn_udot4(w1, cc.c8[t][1], sum, false);
+    sum = __builtin_amdgcn_udot4(w2, cc.c8[t][2], sum, false);
+    const float sf = (float)sum;
+    const float q = rintf(sf * cc.rmf[t]);
+    const int v = (int)fmaf(-q, (float)cc.m[t], sf);
+    r[t] = neg ? -v : v;
   }
 }
 
@@ -802,19 +817,31 @@ __global__ __launch_bounds__(256) void residue_cols_kernel(const double *__restr
 template <int L>
 __device__ __forceinline__ double crt_value(const int (&r)[L], const CrtConsts &cc) {
   int v[L];
+  uint32_t vp[(L + 3) / 4] = {};  // digits packed four to a word
+  // m_0 = 256 and inv_0 = 1: the first digit is the first residue
+  v[0] = r[0];
+  vp[0] = (uint32_t)(r[0] & 0xff);
 #pragma unroll
-  for (int t = 0; t < L; ++t) {
-    // v_t = (r_t - sum_s v_s pmod[s][t]) inv_t mod m_t = r_t inv_t + sum_s v_s qinv[s][t]
-    // mod m_t; |v_s| <= 128 and factors < 256 keep the sum below 2^20 (24-bit products,
-    // one exact FP32 reduction)
-    int y = __mul24(r[t], cc.inv[t]);
+  for (int t = 1; t < L; ++t) {
+    // |r_t inv_t| and each |v_s q[s][t]| are at most 2^14, so |y| < 2^18; m_t is odd, so
+    // y / m_t is at least 1 / (2 m_t) from a half-integer, far beyond the FP32 rounding of
+    // y * (1 / m_t) (< 2^-12): rint gives the exact nearest quotient and the symmetric
+    // residue y - q m_t is exact.
+    int y = r[t] * cc.invs[t];
 #pragma unroll
-    for (int s2 = 0; s2 < t; ++s2) y = __mul24(v[s2], cc.qinv[s2][t]) + y;
-    v[t] = smod_f((float)y, cc.m[t], cc.rmf[t]);
+    for (int w = 0; w < (t + 3) / 4; ++w) y = __builtin_amdgcn_sdot4((int)vp[w], cc.qp[t][w], y, false);
+    const float yf = (float)y;
+    const float q = rintf(yf * cc.rmf[t]);
+    v[t] = (int)fmaf(-q, (float)cc.m[t], yf);
+    vp[t / 4] |= (uint32_t)(v[t] & 0xff) << (8 * (t % 4));
   }
+  // Horner from the top digit: plain FP64 while exact, then double-double
+  constexpr int S = crt_plain_start(L);
   double hi = (double)v[L - 1], lo = 0.0;
 #pragma unroll
-  for (int t = L - 2; t >= 0; --t) {
+  for (int t = L - 2; t >= S; --t) hi = fma(hi, (double)cc.m[t], (double)v[t]);
+#pragma unroll
+  for (int t = S - 1; t >= 0; --t) {
     // (hi, lo) = (hi, lo) * m_t + v_t
     const double mt = (double)cc.m[t];
     const double ph = hi * mt;
@@ -943,13 +970,18 @@ CrtConsts make_crt(int L) {
       pmod[s2] = prod;
       prod = prod * kModuli[s2] % m;
     }
-    c.inv[t] = t == 0 ? 1 : modinv((int)prod, m);
-    for (int s2 = 0; s2 < t; ++s2) c.qinv[s2][t] = (int)((m - pmod[s2] * c.inv[t] % m) % m);
+    const int inv = t == 0 ? 1 : modinv((int)prod, m);
+    auto sym = [m](long x) { return (int)(x > (m - 1) / 2 ? x - m : x); };  // [0, m) -> symmetric
+    c.invs[t] = sym(inv);
+    for (int s2 = 0; s2 < t; ++s2) {
+      const int q = sym((m - pmod[s2] * inv % m) % m);
+      c.qp[t][s2 / 4] |= (int)((uint32_t)(q & 0xff) << (8 * (s2 % 4)));
+    }
     c.rmf[t] = 1.0f / (float)m;
-    long w = 1;
-    for (int i = 0; i < 6; ++i) {
-      c.c12[t][i] = (float)w;
-      for (int b = 0; b < 12; ++b) w = w * 2 % m;
+    long w = 1;  // 2^(8 i) mod m
+    for (int i = 0; i < 9; ++i) {
+      c.c8[t][i / 4] |= (uint32_t)w << (8 * (i % 4));
+      w = w * 256 % m;
     }
   }
   return c;
