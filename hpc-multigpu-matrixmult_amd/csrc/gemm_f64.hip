@@ -1028,7 +1028,7 @@ int launch_transposed(std::vector<GemmDesc> &live, int grid, int config_id, hipS
     for (size_t j = 0; j < srcs.size(); ++j)
       if (srcs[j].A == d.A && srcs[j].lda == d.lda && srcs[j].m == d.m && srcs[j].k == d.k) hit = (int)j;
     if (hit < 0) {
-      const long ldat = ((long)d.m + 1) / 2 * 2;
+      const long ldat = phpc_f64_at_pitch(d.m);
       srcs.push_back({d.A, d.lda, d.m, d.k, ldat, total});
       total += ((size_t)d.k * ldat + 31) / 32 * 32;  // 256-byte aligned copies
       hit = (int)srcs.size() - 1;
@@ -1039,11 +1039,8 @@ int launch_transposed(std::vector<GemmDesc> &live, int grid, int config_id, hipS
   g_at_ws.after_previous(stream);
   double *ws = static_cast<double *>(g_at_ws.ws.get(total * sizeof(double), 0, stream));
   if (!ws) return phpc_set_error(PHPC_ERR_MEM, "transposed-A workspace allocation failed (%zu bytes)", total * 8);
-  for (const Src &c : srcs) {
-    const dim3 tg((unsigned)((c.k + 31) / 32), (unsigned)((c.ldat + 31) / 32));
-    hipLaunchKernelGGL(f64_transpose_kernel, tg, dim3(256), 0, stream, c.A, c.lda, c.m, c.k, ws + c.off, c.ldat);
-    PHPC_HIP(hipGetLastError());
-  }
+  for (const Src &c : srcs)
+    if (int rc = phpc_f64_transpose(c.A, c.lda, c.m, c.k, ws + c.off, c.ldat, stream)) return rc;
   for (size_t i = 0; i < live.size(); ++i) {
     live[i].A = ws + srcs[which[i]].off;
     live[i].lda = srcs[which[i]].ldat;
@@ -1062,6 +1059,35 @@ int phpc_f64_grouped_dev(const GemmDesc *descs_dev, const GemmDesc *descs_host, 
 
 bool phpc_f64_config_trans_a(int id) { return id >= 0 && id < kNumConfigs && kConfigs[id].trans_a; }
 
+long phpc_f64_at_pitch(int m) { return ((long)m + 1) / 2 * 2; }
+
+bool phpc_f64_trans_a_ok(const GemmDesc *descs, int ndesc) {
+  for (int i = 0; i < ndesc; ++i) {
+    const GemmDesc &d = descs[i];
+    if (d.m <= 0 || d.n <= 0 || d.k <= 0) continue;
+    if (!b_vec2_ok(d) || (d.panel > 0 && d.panel < d.k) || phpc_f64_at_pitch(d.m) / 32 >= 65535) return false;
+  }
+  return true;
+}
+
+int phpc_f64_transpose(const double *A, long lda, int m, int k, double *At, long ldat, hipStream_t stream) {
+  if (m <= 0 || k <= 0) return PHPC_OK;
+  const dim3 tg((unsigned)((k + 31) / 32), (unsigned)((ldat + 31) / 32));
+  hipLaunchKernelGGL(f64_transpose_kernel, tg, dim3(256), 0, stream, A, lda, m, k, At, ldat);
+  PHPC_HIP(hipGetLastError());
+  return PHPC_OK;
+}
+
+int phpc_f64_grouped_dev_at(const GemmDesc *descs_dev, const GemmDesc *descs_host, int ndesc, int grid,
+                            int config_id, hipStream_t stream) {
+  return launch_grouped(descs_dev, descs_host, ndesc, grid, config_id, stream, true);
+}
+
+int phpc_f64_pick_config_at(const GemmDesc *descs, int ndesc) {
+  const int cfg = phpc_f64_pick_config(descs, ndesc);
+  return cfg == g_big_single && phpc_f64_trans_a_ok(descs, ndesc) ? g_big_single_host : cfg;
+}
+
 int phpc_f64_grouped(const GemmDesc *descs_in, int ndesc, int grid, int config_id,
                      hipStream_t stream) {
   if (ndesc <= 0) return PHPC_OK;
@@ -1070,22 +1096,13 @@ int phpc_f64_grouped(const GemmDesc *descs_in, int ndesc, int grid, int config_i
     if (d.m < 0 || d.n < 0 || d.k < 0) return phpc_set_error(PHPC_ERR_ARG, "negative GEMM size");
   }
   std::vector<GemmDesc> descs(descs_in, descs_in + ndesc);
-  if (config_id < 0) {
-    config_id = phpc_f64_pick_config(descs.data(), ndesc);
-    if (config_id == g_big_single) config_id = g_big_single_host;
-  }
+  if (config_id < 0) config_id = phpc_f64_pick_config_at(descs.data(), ndesc);
   if (config_id < 0 || config_id >= kNumConfigs)
     return phpc_set_error(PHPC_ERR_ARG, "bad FP64 config id %d", config_id);
-  if (kConfigs[config_id].trans_a) {
-    // the transposed copy needs 16-byte B rows, one panel and a grid-sized m; otherwise
-    // the same tile runs register-staged on A as it is
-    bool ok = true;
-    for (const GemmDesc &d : descs)
-      if (d.m > 0 && d.n > 0 && d.k > 0 &&
-          (!b_vec2_ok(d) || (d.panel > 0 && d.panel < d.k) || ((long)d.m + 1) / 2 * 2 / 32 >= 65535))
-        ok = false;
-    if (!ok) config_id = kConfigs[config_id].bm == 256 ? 6 : 8;
-  }
+  // the transposed copy needs 16-byte B rows, one panel and a grid-sized m; otherwise the
+  // same tile runs register-staged on A as it is
+  if (kConfigs[config_id].trans_a && !phpc_f64_trans_a_ok(descs.data(), ndesc))
+    config_id = kConfigs[config_id].bm == 256 ? 6 : 8;
   const ConfigEntry &ce = kConfigs[config_id];
   // drop empty GEMMs; k == 0 leaves C unchanged (C += 0 adds +0.0 only to tiles we skip)
   std::vector<GemmDesc> live;

@@ -20,6 +20,18 @@ struct phpc_plan {
   int total_tiles = 0;
   int grid = 0;
   double tuned_ms = 0.0;
+  // transposed-A configurations: one transposed copy per distinct A block (plan-owned),
+  // and the descriptors that point at the copies
+  struct AtSrc {
+    const double *A;
+    long lda;
+    int m, k;
+    long ldat;
+    size_t off;  // doubles into at_ws
+  };
+  std::vector<AtSrc> at_srcs;
+  std::vector<GemmDesc> d64_at;
+  double *at_ws = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t graph_stream = nullptr;
@@ -46,11 +58,52 @@ std::string signature(int dtype, const phpc_gemm_item *it, int n) {
   return s;
 }
 
+// Transposed-A copies for a plan: allocated once (the plan keeps them), filled by every
+// launch before the GEMM kernel.
+int plan_prepare_at(phpc_plan *p) {
+  if (p->at_ws) return PHPC_OK;
+  size_t total = 0;
+  p->d64_at = p->d64;
+  for (auto &d : p->d64_at) {
+    int hit = -1;
+    for (size_t j = 0; j < p->at_srcs.size(); ++j) {
+      const auto &c = p->at_srcs[j];
+      if (c.A == d.A && c.lda == d.lda && c.m == d.m && c.k == d.k) hit = (int)j;
+    }
+    if (hit < 0) {
+      const long ldat = phpc_f64_at_pitch(d.m);
+      p->at_srcs.push_back({d.A, d.lda, d.m, d.k, ldat, total});
+      total += ((size_t)d.k * ldat + 31) / 32 * 32;
+      hit = (int)p->at_srcs.size() - 1;
+    }
+    d.lda = p->at_srcs[hit].ldat;
+    d.A = nullptr;  // set below, once the workspace exists
+  }
+  if (hipMalloc(&p->at_ws, total * sizeof(double)) != hipSuccess) {
+    p->at_ws = nullptr;
+    p->at_srcs.clear();
+    p->d64_at.clear();
+    return phpc_set_error(PHPC_ERR_MEM, "plan transposed-A workspace allocation failed (%zu bytes)", total * 8);
+  }
+  for (size_t i = 0; i < p->d64_at.size(); ++i) {
+    const GemmDesc &d = p->d64[i];
+    for (const auto &c : p->at_srcs)
+      if (c.A == d.A && c.lda == d.lda && c.m == d.m && c.k == d.k) p->d64_at[i].A = p->at_ws + c.off;
+  }
+  return PHPC_OK;
+}
+
 // (re)plan tiles for config id and upload the descriptors
 int plan_upload(phpc_plan *p, int id, hipStream_t s) {
   size_t bytes;
   const void *src;
-  if (p->dtype == PHPC_F64) {
+  if (p->dtype == PHPC_F64 && phpc_f64_config_trans_a(id)) {
+    if (int rc = plan_prepare_at(p)) return rc;
+    p->total_tiles = phpc_f64_plan_tiles(p->d64_at.data(), p->count, id);
+    phpc_f64_plan_tiles(p->d64.data(), p->count, id);
+    bytes = sizeof(GemmDesc) * p->count;
+    src = p->d64_at.data();
+  } else if (p->dtype == PHPC_F64) {
     p->total_tiles = phpc_f64_plan_tiles(p->d64.data(), p->count, id);
     bytes = sizeof(GemmDesc) * p->count;
     src = p->d64.data();
@@ -69,6 +122,12 @@ int plan_upload(phpc_plan *p, int id, hipStream_t s) {
 
 int launch(phpc_plan *p, hipStream_t s) {
   if (p->count == 0 || p->total_tiles == 0) return PHPC_OK;
+  if (p->dtype == PHPC_F64 && phpc_f64_config_trans_a(p->config)) {
+    for (const auto &c : p->at_srcs)
+      if (int rc = phpc_f64_transpose(c.A, c.lda, c.m, c.k, p->at_ws + c.off, c.ldat, s)) return rc;
+    return phpc_f64_grouped_dev_at((const GemmDesc *)p->descs_dev, p->d64_at.data(), p->count, p->grid, p->config,
+                                   s);
+  }
   if (p->dtype == PHPC_F64)
     return phpc_f64_grouped_dev((const GemmDesc *)p->descs_dev, p->d64.data(), p->count, p->grid, p->config, s);
   if (p->dtype == PHPC_F32X3)
@@ -142,8 +201,10 @@ extern "C" int phpc_plan_create(int dtype, const phpc_gemm_item *items, int coun
     delete p;
     return phpc_set_error(PHPC_ERR_MEM, "descriptor allocation failed");
   }
-  // default choice: the planner's rule
-  int cfg = dtype == PHPC_F64 ? phpc_f64_pick_config(p->d64.data(), p->count) : 0;
+  // default choice: the planner's rule; large single-panel FP64 work takes the
+  // transposed-A kernel when B allows 16-byte loads (the plan then keeps a copy of A)
+  int cfg = dtype == PHPC_F64 ? phpc_f64_pick_config_at(p->d64.data(), p->count) : 0;
+  const bool at_ok = dtype == PHPC_F64 && phpc_f64_trans_a_ok(p->d64.data(), p->count);
   if (dtype == PHPC_F32) {
     long work = 0;
     for (const auto &d : p->d32) work += (long)((d.m + 127) / 128) * ((d.n + 127) / 128);
@@ -183,7 +244,7 @@ extern "C" int phpc_plan_create(int dtype, const phpc_gemm_item *items, int coun
       (void)hipEventCreate(&e1);
       for (int c = 0; c < ncfg; ++c) {
         if (dtype == PHPC_F64 && mp && !phpc_f64_config_panels(c)) continue;
-        if (dtype == PHPC_F64 && phpc_f64_config_trans_a(c)) continue;
+        if (dtype == PHPC_F64 && phpc_f64_config_trans_a(c) && !at_ok) continue;
         if (dtype == PHPC_F32X3 && c != 2) continue;
         if (plan_upload(p, c, s) != PHPC_OK) continue;
         p->grid = 0;
@@ -272,5 +333,6 @@ extern "C" void phpc_plan_destroy(phpc_plan_t p) {
     (void)hipEventDestroy(kv.second);
   }
   if (p->descs_dev) (void)hipFree(p->descs_dev);
+  if (p->at_ws) (void)hipFree(p->at_ws);
   delete p;
 }

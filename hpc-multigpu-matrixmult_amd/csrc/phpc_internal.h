@@ -113,9 +113,19 @@ int phpc_f64_grouped_dev(const GemmDesc *descs_dev, const GemmDesc *descs_host, 
 int phpc_f64_num_configs();
 F64Config phpc_f64_config(int id);
 int phpc_f64_pick_config(const GemmDesc *descs, int ndesc);
-// Configurations whose kernels read a transposed copy of A, which only phpc_f64_grouped
-// (host descriptors) stages; phpc_f64_grouped_dev rejects them.
+// Configurations whose kernels read a transposed copy of A (descriptor A = At[k][m] with
+// lda = phpc_f64_at_pitch(m), pad column zero). phpc_f64_grouped stages the copy itself;
+// callers with device descriptors stage it with phpc_f64_transpose and launch through
+// phpc_f64_grouped_dev_at (phpc_f64_grouped_dev rejects these configurations).
 bool phpc_f64_config_trans_a(int id);
+bool phpc_f64_trans_a_ok(const GemmDesc *descs, int ndesc);
+long phpc_f64_at_pitch(int m);
+int phpc_f64_transpose(const double *A, long lda, int m, int k, double *At, long ldat, hipStream_t stream);
+int phpc_f64_grouped_dev_at(const GemmDesc *descs_dev, const GemmDesc *descs_host, int ndesc, int grid,
+                            int config_id, hipStream_t stream);
+// the planner's choice, with the transposed-A configuration for large single-panel work
+// whose B allows it
+int phpc_f64_pick_config_at(const GemmDesc *descs, int ndesc);
 
 // Strided block pack: dst[r*ldd + c] = src[r*lds + c] for r < rows, c < cols.
 // Host <-> device copies through pinned staging buffers (rows x width bytes, strided).

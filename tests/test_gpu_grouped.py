@@ -149,3 +149,35 @@ def test_f64_n10000_grid3x3_one_launch():
     cols = np.concatenate([rng.integers(0, N, 1024), [3333, 3334, 6666, 6667, 9999]])
     cw, _ = phpc.check_sampled(dA, dB, dC, rows, cols)
     assert cw <= 1e-14
+
+
+def test_f64_plan_transposed_a_graph_replays():
+    """Large single-panel FP64 plans take the transposed-A kernel: the plan keeps its own
+    transposed copies of A, refreshed by every launch, and captured replays (eager work and
+    synchronisations in between) equal eager launches of the same plan bit for bit; the
+    result also equals the host-descriptor entry point's."""
+    import torch
+    N = 4096
+    dA = torch.empty((N, N), dtype=torch.float64, device="cuda")
+    dB = torch.empty_like(dA)
+    phpc.fill_device(dA, phpc.DIST_UNIFORM, 11)
+    phpc.fill_device(dB, phpc.DIST_UNIFORM, 12)
+    dCg, dCe, dCd = torch.zeros_like(dA), torch.zeros_like(dA), torch.zeros_like(dA)
+    s = torch.cuda.Stream()
+    pg = phpc.Plan(phpc.F64, summa_items(dA, dB, dCg, N, 2, 2, 0))
+    pe = phpc.Plan(phpc.F64, summa_items(dA, dB, dCe, N, 2, 2, 0))
+    assert pg.info()["config"] == 11
+    junk = torch.randn(1000, 1000, device="cuda")
+    for rep in range(3):
+        with torch.cuda.stream(s):
+            pg.graph_launch(s)
+        s.synchronize()
+        junk = junk @ junk.T / 1000.0
+        torch.cuda.synchronize()
+        pe.launch()
+        phpc.dgemm_device(dA, dB, dCd)
+        torch.cuda.synchronize()
+        assert torch.equal(dCg.view(torch.int64), dCe.view(torch.int64)), rep
+        assert torch.equal(dCg.view(torch.int64), dCd.view(torch.int64)), rep
+    pg.close()
+    pe.close()
