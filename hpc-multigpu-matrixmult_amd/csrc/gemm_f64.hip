@@ -613,7 +613,8 @@ __global__ __launch_bounds__(WM *WN * 64) __attribute__((amdgpu_waves_per_eu(2))
 // one contiguous 1 KiB piece of a k row. LDS rows are padded by 64 bytes. The MFMA
 // operand reads are the register-staged kernel's (16 consecutive doubles per lane
 // group), so the arithmetic is identical. Two stages: a slab's loads land while the
-// previous slab is computed. One panel only.
+// previous slab is computed. MP: k is walked in panels, each panel's sum starts from zero
+// and is added to the C value carried in registers (as the register-staged MP kernels).
 // ---------------------------------------------------------------------------
 template <int BM, int BN, int WM, int WN>
 struct F64At {
@@ -632,7 +633,7 @@ struct F64At {
   static_assert(STAGES * STAGE_BYTES <= 160 * 1024, "LDS budget");
 };
 
-template <int BM, int BN, int WM, int WN>
+template <int BM, int BN, int WM, int WN, bool MP>
 __device__ __forceinline__ void f64_at_body(const GemmBatch &batch) {
   using K = F64At<BM, BN, WM, WN>;
   extern __shared__ __attribute__((aligned(1024))) char smem[];
@@ -670,7 +671,12 @@ __device__ __forceinline__ void f64_at_body(const GemmBatch &batch) {
     const double *Bg = uni(d.B) + col0;
     gdouble *Cg = (gdouble *)(uni(d.C) + (long)row0 * ldc + col0);
     const int Kdim = uni(d.k);
-    const int nslab = (Kdim + K::BK - 1) / K::BK;
+    const int dpanel = uni(d.panel);
+    const int w = MP && dpanel > 0 && dpanel < Kdim ? dpanel : Kdim;
+    const int spp = (w + K::BK - 1) / K::BK;  // slabs per full panel
+    const int full_panels = Kdim / w;
+    const int tail = Kdim - full_panels * w;
+    const int nslab = full_panels * spp + (tail + K::BK - 1) / K::BK;
 
     // lane offsets of this wave's pieces within a slab (k row kr, columns c, c+1);
     // columns past the tile's edge read zero
@@ -696,9 +702,16 @@ __device__ __forceinline__ void f64_at_body(const GemmBatch &batch) {
     // slab bases advance by BK rows; a slab's byte range is BK rows (fewer for the last
     // slab, whose missing k rows then read zero). The launcher keeps BK rows < 2 GiB.
     const int full_a = (int)(K::BK * ldat * 8), full_b = (int)(K::BK * ldb * 8);
+    // slab sidx: k rows [kb, kb + krem) of panel p (slabs never cross a panel's end)
+    auto slab_rows = [&](int sidx, int &kb, int &krem, int &pe) {
+      const int p = sidx / spp;
+      kb = p * w + (sidx - p * spp) * K::BK;
+      pe = min(p * w + w, Kdim);
+      krem = min(K::BK, pe - kb);
+    };
     auto issue = [&](int sidx) {
-      const int kb = sidx * K::BK;
-      const int krem = min(K::BK, Kdim - kb);
+      int kb, krem, pe;
+      slab_rows(sidx, kb, krem, pe);
       char *stage = smem + (sidx % K::STAGES) * K::STAGE_BYTES;
       const int na = krem == K::BK ? full_a : krem * (int)ldat * 8;
       const int nb = krem == K::BK ? full_b : krem * (int)ldb * 8;
@@ -715,10 +728,24 @@ __device__ __forceinline__ void f64_at_body(const GemmBatch &batch) {
     };
 
     d4 acc[K::TM][K::TN];
+    d4 creg[MP ? K::TM : 1][MP ? K::TN : 1];
 #pragma unroll
     for (int i = 0; i < K::TM; ++i)
 #pragma unroll
       for (int j = 0; j < K::TN; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+    if constexpr (MP) {
+#pragma unroll
+      for (int i = 0; i < K::TM; ++i)
+#pragma unroll
+        for (int j = 0; j < K::TN; ++j) {
+          const int cc = wn * K::WTN + j * 16 + li;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int rr = wm * K::WTM + i * 16 + 4 * r + g;
+            creg[i][j][r] = (rr < m_rem && cc < n_rem) ? Cg[(long)rr * ldc + cc] : 0.0;
+          }
+        }
+    }
 
     // every wave must be done reading the previous tile's stages before refilling them
     __syncthreads();
@@ -748,6 +775,19 @@ __device__ __forceinline__ void f64_at_body(const GemmBatch &batch) {
       if (sidx + 1 < nslab) issue(sidx + 1);
 #pragma unroll
       for (int kk = 1; kk < K::BK / 4; ++kk) kstep(kk);
+      if constexpr (MP) {
+        int kb, krem, pe;
+        slab_rows(sidx, kb, krem, pe);
+        if (kb + krem == pe) {
+#pragma unroll
+          for (int i = 0; i < K::TM; ++i)
+#pragma unroll
+            for (int j = 0; j < K::TN; ++j) {
+              creg[i][j] = creg[i][j] + acc[i][j];
+              acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+            }
+        }
+      }
     }
 
 #pragma unroll
@@ -760,7 +800,10 @@ __device__ __forceinline__ void f64_at_body(const GemmBatch &batch) {
           const int rr = wm * K::WTM + i * 16 + 4 * r + g;
           if (rr < m_rem && cc < n_rem) {
             gdouble *pc = Cg + (long)rr * ldc + cc;
-            *pc = *pc + acc[i][jj][r];
+            if constexpr (MP)
+              *pc = creg[i][jj][r];
+            else
+              *pc = *pc + acc[i][jj][r];
           }
         }
       }
@@ -768,10 +811,17 @@ __device__ __forceinline__ void f64_at_body(const GemmBatch &batch) {
 }
 
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void f64_at_256x128(GemmBatch batch) {
-  f64_at_body<256, 128, 4, 2>(batch);
+  f64_at_body<256, 128, 4, 2, false>(batch);
 }
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void f64_at_128x128(GemmBatch batch) {
-  f64_at_body<128, 128, 2, 2>(batch);
+  f64_at_body<128, 128, 2, 2, false>(batch);
+}
+// 8 waves with 64x32 wave tiles, so the carried C fits beside the accumulators
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void f64_at_128x128_w8(GemmBatch batch) {
+  f64_at_body<128, 128, 2, 4, false>(batch);
+}
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void f64_at_128x128_w8_mp(GemmBatch batch) {
+  f64_at_body<128, 128, 2, 4, true>(batch);
 }
 
 // At[k][m] = A[m][k] for m < M, k < K; At[k][m] = 0 for M <= m < ldat (row pitch ldat)
@@ -867,6 +917,12 @@ const ConfigEntry kConfigs[] = {
      {{PHPC_V1(128, 128, 2, 2, 16, false, 1), f64_at_128x128},
       {PHPC_V1(128, 128, 2, 2, 16, false, 1), f64_at_128x128}},
      true},
+    // 12: transposed-A direct-to-LDS, 8 waves (wave tile 64x32), 128x128 tiles, 2 stages,
+    //     panel-grouped too (C carried in registers). 8-byte slots: config 0's kernels.
+    {128, 128, 512, 16, {0, 2 * (size_t)F64At<128, 128, 2, 4>::STAGE_BYTES}, true,
+     {{PHPC_V1(128, 128, 2, 4, 16, false, 1), f64_at_128x128_w8},
+      {PHPC_V1(128, 128, 2, 4, 16, true, 1), f64_at_128x128_w8_mp}},
+     true},
 };
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 
@@ -955,6 +1011,7 @@ namespace {
 // Configuration for large single-panel work reached through host descriptors
 // (phpc_f64_grouped), which can stage a transposed copy of A first.
 int g_big_single_host = 11;
+int g_big_config_host = 12;  // large panel-grouped work through host descriptors
 
 bool b_vec2_ok(const GemmDesc &d) { return aligned16(d.B) && (d.ldb % 2 == 0) && (d.n % 2 == 0); }
 
@@ -1065,7 +1122,7 @@ bool phpc_f64_trans_a_ok(const GemmDesc *descs, int ndesc) {
   for (int i = 0; i < ndesc; ++i) {
     const GemmDesc &d = descs[i];
     if (d.m <= 0 || d.n <= 0 || d.k <= 0) continue;
-    if (!b_vec2_ok(d) || (d.panel > 0 && d.panel < d.k) || phpc_f64_at_pitch(d.m) / 32 >= 65535) return false;
+    if (!b_vec2_ok(d) || phpc_f64_at_pitch(d.m) / 32 >= 65535) return false;
   }
   return true;
 }
@@ -1085,7 +1142,10 @@ int phpc_f64_grouped_dev_at(const GemmDesc *descs_dev, const GemmDesc *descs_hos
 
 int phpc_f64_pick_config_at(const GemmDesc *descs, int ndesc) {
   const int cfg = phpc_f64_pick_config(descs, ndesc);
-  return cfg == g_big_single && phpc_f64_trans_a_ok(descs, ndesc) ? g_big_single_host : cfg;
+  if (!phpc_f64_trans_a_ok(descs, ndesc)) return cfg;
+  if (cfg == g_big_single) return g_big_single_host;
+  if (cfg == g_big_config) return g_big_config_host;
+  return cfg;
 }
 
 int phpc_f64_grouped(const GemmDesc *descs_in, int ndesc, int grid, int config_id,
@@ -1099,10 +1159,10 @@ int phpc_f64_grouped(const GemmDesc *descs_in, int ndesc, int grid, int config_i
   if (config_id < 0) config_id = phpc_f64_pick_config_at(descs.data(), ndesc);
   if (config_id < 0 || config_id >= kNumConfigs)
     return phpc_set_error(PHPC_ERR_ARG, "bad FP64 config id %d", config_id);
-  // the transposed copy needs 16-byte B rows, one panel and a grid-sized m; otherwise the
-  // same tile runs register-staged on A as it is
+  // the transposed copy needs 16-byte B rows and a grid-sized m; otherwise the same tile
+  // runs register-staged on A as it is
   if (kConfigs[config_id].trans_a && !phpc_f64_trans_a_ok(descs.data(), ndesc))
-    config_id = kConfigs[config_id].bm == 256 ? 6 : 8;
+    config_id = kConfigs[config_id].bm == 256 ? 6 : kConfigs[config_id].panels ? 0 : 8;
   const ConfigEntry &ce = kConfigs[config_id];
   // drop empty GEMMs; k == 0 leaves C unchanged (C += 0 adds +0.0 only to tiles we skip)
   std::vector<GemmDesc> live;
@@ -1160,7 +1220,8 @@ int launch_live(std::vector<GemmDesc> &live, int grid, int config_id, hipStream_
 // Tuning knobs for the autotuner: 0 = config for large panel-grouped problems, 1 =
 // tile-row swizzle width, 2 = XCD-contiguous tile order on/off, 3 = config for large
 // single-panel problems on device descriptors, 4 = the same through host descriptors
-// (may be a transposed-A configuration).
+// (may be a transposed-A configuration), 5 = large panel-grouped problems through host
+// descriptors.
 extern "C" int phpc_set_tuning(int key, int value) {
   switch (key) {
     case 0:
@@ -1183,6 +1244,11 @@ extern "C" int phpc_set_tuning(int key, int value) {
     case 4:
       if (value < 0 || value >= kNumConfigs) return phpc_set_error(PHPC_ERR_ARG, "config %d out of range", value);
       g_big_single_host = value;
+      break;
+    case 5:
+      if (value < 0 || value >= kNumConfigs || !kConfigs[value].panels)
+        return phpc_set_error(PHPC_ERR_ARG, "config %d out of range", value);
+      g_big_config_host = value;
       break;
     default:
       return phpc_set_error(PHPC_ERR_ARG, "unknown tuning key %d", key);

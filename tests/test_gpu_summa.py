@@ -33,6 +33,26 @@ def test_summa_device_modes_match_oracle(p, q, N, panel):
         assert np.array_equal(bits(dC.cpu().numpy()), bits(want)), mode
 
 
+@pytest.mark.parametrize("cfg", [0, 3, 12])
+@pytest.mark.parametrize("p,q,N,panel", [(4, 4, 512, 32), (2, 3, 257, 16), (3, 3, 300, 7), (2, 2, 256, 0)])
+def test_summa_fused_panel_configs_match_oracle(cfg, p, q, N, panel):
+    """FUSED on each panel-grouped tile configuration, the transposed-A one (12) included,
+    is bit-exact against the oracle's SUMMA emulation."""
+    import torch
+    print(f"fused cfg={cfg} p={p} q={q} N={N} panel={panel}", flush=True)
+    A, B, C0 = O.uniform((N, N), 21), O.uniform((N, N), 22), O.uniform((N, N), 23)
+    want = O.summa_emul(A, B, C0.copy(), p, q, panel)
+    dC = torch.from_numpy(C0.copy()).cuda()
+    phpc.set_launch_policy(0, cfg, 0)
+    try:
+        phpc.summa_device(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(), dC, p, q, panel=panel,
+                          mode=phpc.SUMMA_FUSED)
+        torch.cuda.synchronize()
+    finally:
+        phpc.set_launch_policy(0, -1, 0)
+    assert np.array_equal(bits(dC.cpu().numpy()), bits(want))
+
+
 @pytest.mark.parametrize("P,N", [(1, 128), (4, 256), (16, 512), (6, 300)])
 def test_summa_cuda_host_entry_point(P, N):
     A, B, C0 = O.uniform((N, N), 4), O.uniform((N, N), 5), O.uniform((N, N), 6)
