@@ -843,6 +843,14 @@ __global__ __launch_bounds__(256) void f64_transpose_kernel(const double *__rest
   }
 }
 
+// Bc[k][j] = B[k][j] for j < N, 0 for N <= j < ldbc (row pitch ldbc)
+__global__ __launch_bounds__(256) void f64_copy_pad_kernel(const double *__restrict__ B, long ldb, int Kd, int N,
+                                                           double *__restrict__ Bc, long ldbc) {
+  const int k = blockIdx.y;
+  for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < ldbc; j += (long)gridDim.x * blockDim.x)
+    Bc[(long)k * ldbc + j] = j < N ? B[(long)k * ldb + j] : 0.0;
+}
+
 typedef void (*KernelFn)(GemmBatch);
 
 // One tile configuration: kernels for [panel grouping][16-byte loads]. Direct-to-LDS
@@ -1014,6 +1022,10 @@ int g_big_single_host = 11;
 int g_big_config_host = 12;  // large panel-grouped work through host descriptors
 
 bool b_vec2_ok(const GemmDesc &d) { return aligned16(d.B) && (d.ldb % 2 == 0) && (d.n % 2 == 0); }
+// the transposed-A kernels load B in 16-byte column pairs; with an even ldb and an aligned
+// base, the pair holding an odd n's last column ends inside the row, and the extra column
+// only feeds an output column that is never stored
+bool b_at_ok(const GemmDesc &d) { return aligned16(d.B) && (d.ldb % 2 == 0); }
 
 int launch_grouped(const GemmDesc *descs_dev, const GemmDesc *descs_host, int ndesc, int grid, int config_id,
                    hipStream_t stream, bool a_transposed) {
@@ -1036,7 +1048,7 @@ int launch_grouped(const GemmDesc *descs_dev, const GemmDesc *descs_host, int nd
     // is stored that way by construction)
     const bool a_ok = a_transposed || (aligned16(d.A) && (d.lda % 2 == 0) && (d.k % 2 == 0) &&
                                        (d.panel <= 0 || d.panel >= d.k || d.panel % 2 == 0));
-    if (!a_ok || !b_vec2_ok(d)) vec2 = false;
+    if (!a_ok || !(a_transposed ? b_at_ok(d) : b_vec2_ok(d))) vec2 = false;
     total = d.tile_start + ((d.m + ce.bm - 1) / ce.bm) * d.tiles_n;
   }
   if (a_transposed && !vec2) return phpc_set_error(PHPC_ERR_ARG, "transposed-A kernels need 16-byte B rows");
@@ -1069,39 +1081,14 @@ SharedWorkspace g_at_ws;
 int launch_live(std::vector<GemmDesc> &live, int grid, int config_id, hipStream_t stream, bool a_transposed);
 
 int launch_transposed(std::vector<GemmDesc> &live, int grid, int config_id, hipStream_t stream) {
-  struct Src {
-    const double *A;
-    long lda;
-    int m, k;
-    long ldat;
-    size_t off;  // doubles
-  };
-  std::vector<Src> srcs;
-  size_t total = 0;
-  std::vector<int> which(live.size());
-  for (size_t i = 0; i < live.size(); ++i) {
-    const GemmDesc &d = live[i];
-    int hit = -1;
-    for (size_t j = 0; j < srcs.size(); ++j)
-      if (srcs[j].A == d.A && srcs[j].lda == d.lda && srcs[j].m == d.m && srcs[j].k == d.k) hit = (int)j;
-    if (hit < 0) {
-      const long ldat = phpc_f64_at_pitch(d.m);
-      srcs.push_back({d.A, d.lda, d.m, d.k, ldat, total});
-      total += ((size_t)d.k * ldat + 31) / 32 * 32;  // 256-byte aligned copies
-      hit = (int)srcs.size() - 1;
-    }
-    which[i] = hit;
-  }
+  F64Staging st;
+  st.plan(live.data(), (int)live.size());
   std::lock_guard<std::mutex> lk(g_at_ws.mu);
   g_at_ws.after_previous(stream);
-  double *ws = static_cast<double *>(g_at_ws.ws.get(total * sizeof(double), 0, stream));
-  if (!ws) return phpc_set_error(PHPC_ERR_MEM, "transposed-A workspace allocation failed (%zu bytes)", total * 8);
-  for (const Src &c : srcs)
-    if (int rc = phpc_f64_transpose(c.A, c.lda, c.m, c.k, ws + c.off, c.ldat, stream)) return rc;
-  for (size_t i = 0; i < live.size(); ++i) {
-    live[i].A = ws + srcs[which[i]].off;
-    live[i].lda = srcs[which[i]].ldat;
-  }
+  double *ws = static_cast<double *>(g_at_ws.ws.get(st.total * sizeof(double), 0, stream));
+  if (!ws) return phpc_set_error(PHPC_ERR_MEM, "transposed-A workspace allocation failed (%zu bytes)", st.total * 8);
+  if (int rc = st.run(ws, stream)) return rc;
+  st.apply(ws, live.data(), (int)live.size());
   const int rc = launch_live(live, grid, config_id, stream, true);
   g_at_ws.mark_used(stream);
   return rc;
@@ -1122,9 +1109,61 @@ bool phpc_f64_trans_a_ok(const GemmDesc *descs, int ndesc) {
   for (int i = 0; i < ndesc; ++i) {
     const GemmDesc &d = descs[i];
     if (d.m <= 0 || d.n <= 0 || d.k <= 0) continue;
-    if (!b_vec2_ok(d) || phpc_f64_at_pitch(d.m) / 32 >= 65535) return false;
+    // grid limits of the staging kernels
+    if (phpc_f64_at_pitch(d.m) / 32 >= 65535 || (!b_at_ok(d) && d.k >= 65535)) return false;
   }
   return true;
+}
+
+void F64Staging::plan(const GemmDesc *descs, int ndesc) {
+  copies.clear();
+  a_copy.assign(ndesc, -1);
+  b_copy.assign(ndesc, -1);
+  total = 0;
+  auto find_or_add = [&](const double *src, long ld, int rows, int cols, bool tr) {
+    for (size_t j = 0; j < copies.size(); ++j) {
+      const Copy &c = copies[j];
+      if (c.src == src && c.ld == ld && c.rows == rows && c.cols == cols && c.transpose == tr) return (int)j;
+    }
+    const long pitch = phpc_f64_at_pitch(tr ? rows : cols);
+    const size_t n = (size_t)(tr ? cols : rows) * pitch;
+    copies.push_back({src, ld, rows, cols, pitch, total, tr});
+    total += (n + 31) / 32 * 32;  // 256-byte aligned copies
+    return (int)copies.size() - 1;
+  };
+  for (int i = 0; i < ndesc; ++i) {
+    const GemmDesc &d = descs[i];
+    if (d.m <= 0 || d.n <= 0 || d.k <= 0) continue;
+    a_copy[i] = find_or_add(d.A, d.lda, d.m, d.k, true);
+    if (!b_at_ok(d)) b_copy[i] = find_or_add(d.B, d.ldb, d.k, d.n, false);
+  }
+}
+
+int F64Staging::run(double *ws, hipStream_t stream) const {
+  for (const Copy &c : copies) {
+    if (c.transpose) {
+      if (int rc = phpc_f64_transpose(c.src, c.ld, c.rows, c.cols, ws + c.off, c.pitch, stream)) return rc;
+    } else {
+      const dim3 grid((unsigned)std::min<long>((c.pitch + 255) / 256, 64), (unsigned)c.rows);
+      hipLaunchKernelGGL(f64_copy_pad_kernel, grid, dim3(256), 0, stream, c.src, c.ld, c.rows, c.cols, ws + c.off,
+                         c.pitch);
+      PHPC_HIP(hipGetLastError());
+    }
+  }
+  return PHPC_OK;
+}
+
+void F64Staging::apply(double *ws, GemmDesc *descs, int ndesc) const {
+  for (int i = 0; i < ndesc; ++i) {
+    if (a_copy[i] >= 0) {
+      descs[i].A = ws + copies[a_copy[i]].off;
+      descs[i].lda = copies[a_copy[i]].pitch;
+    }
+    if (b_copy[i] >= 0) {
+      descs[i].B = ws + copies[b_copy[i]].off;
+      descs[i].ldb = copies[b_copy[i]].pitch;
+    }
+  }
 }
 
 int phpc_f64_transpose(const double *A, long lda, int m, int k, double *At, long ldat, hipStream_t stream) {

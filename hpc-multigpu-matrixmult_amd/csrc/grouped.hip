@@ -20,16 +20,9 @@ struct phpc_plan {
   int total_tiles = 0;
   int grid = 0;
   double tuned_ms = 0.0;
-  // transposed-A configurations: one transposed copy per distinct A block (plan-owned),
-  // and the descriptors that point at the copies
-  struct AtSrc {
-    const double *A;
-    long lda;
-    int m, k;
-    long ldat;
-    size_t off;  // doubles into at_ws
-  };
-  std::vector<AtSrc> at_srcs;
+  // transposed-A configurations: the operand copies they read (plan-owned workspace) and
+  // the descriptors that point at them
+  F64Staging staging;
   std::vector<GemmDesc> d64_at;
   double *at_ws = nullptr;
   hipGraph_t graph = nullptr;
@@ -62,34 +55,14 @@ std::string signature(int dtype, const phpc_gemm_item *it, int n) {
 // launch before the GEMM kernel.
 int plan_prepare_at(phpc_plan *p) {
   if (p->at_ws) return PHPC_OK;
-  size_t total = 0;
-  p->d64_at = p->d64;
-  for (auto &d : p->d64_at) {
-    int hit = -1;
-    for (size_t j = 0; j < p->at_srcs.size(); ++j) {
-      const auto &c = p->at_srcs[j];
-      if (c.A == d.A && c.lda == d.lda && c.m == d.m && c.k == d.k) hit = (int)j;
-    }
-    if (hit < 0) {
-      const long ldat = phpc_f64_at_pitch(d.m);
-      p->at_srcs.push_back({d.A, d.lda, d.m, d.k, ldat, total});
-      total += ((size_t)d.k * ldat + 31) / 32 * 32;
-      hit = (int)p->at_srcs.size() - 1;
-    }
-    d.lda = p->at_srcs[hit].ldat;
-    d.A = nullptr;  // set below, once the workspace exists
-  }
-  if (hipMalloc(&p->at_ws, total * sizeof(double)) != hipSuccess) {
+  p->staging.plan(p->d64.data(), p->count);
+  if (hipMalloc(&p->at_ws, p->staging.total * sizeof(double)) != hipSuccess) {
     p->at_ws = nullptr;
-    p->at_srcs.clear();
-    p->d64_at.clear();
-    return phpc_set_error(PHPC_ERR_MEM, "plan transposed-A workspace allocation failed (%zu bytes)", total * 8);
+    return phpc_set_error(PHPC_ERR_MEM, "plan transposed-A workspace allocation failed (%zu bytes)",
+                          p->staging.total * 8);
   }
-  for (size_t i = 0; i < p->d64_at.size(); ++i) {
-    const GemmDesc &d = p->d64[i];
-    for (const auto &c : p->at_srcs)
-      if (c.A == d.A && c.lda == d.lda && c.m == d.m && c.k == d.k) p->d64_at[i].A = p->at_ws + c.off;
-  }
+  p->d64_at = p->d64;
+  p->staging.apply(p->at_ws, p->d64_at.data(), p->count);
   return PHPC_OK;
 }
 
@@ -123,8 +96,7 @@ int plan_upload(phpc_plan *p, int id, hipStream_t s) {
 int launch(phpc_plan *p, hipStream_t s) {
   if (p->count == 0 || p->total_tiles == 0) return PHPC_OK;
   if (p->dtype == PHPC_F64 && phpc_f64_config_trans_a(p->config)) {
-    for (const auto &c : p->at_srcs)
-      if (int rc = phpc_f64_transpose(c.A, c.lda, c.m, c.k, p->at_ws + c.off, c.ldat, s)) return rc;
+    if (int rc = p->staging.run(p->at_ws, s)) return rc;
     return phpc_f64_grouped_dev_at((const GemmDesc *)p->descs_dev, p->d64_at.data(), p->count, p->grid, p->config,
                                    s);
   }

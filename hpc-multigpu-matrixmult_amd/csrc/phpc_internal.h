@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <mutex>
+#include <vector>
 #include <stdint.h>
 #include <stddef.h>
 
@@ -121,6 +122,27 @@ bool phpc_f64_config_trans_a(int id);
 bool phpc_f64_trans_a_ok(const GemmDesc *descs, int ndesc);
 long phpc_f64_at_pitch(int m);
 int phpc_f64_transpose(const double *A, long lda, int m, int k, double *At, long ldat, hipStream_t stream);
+
+// Operand copies that the transposed-A kernels read: At[k][m] for every distinct A block,
+// and, for a B block whose rows are not 16-byte aligned (base or odd ldb), a copy with an
+// even pitch. `plan` lists them and sizes the workspace; `run` fills a workspace; `apply`
+// points descriptors at the copies.
+struct F64Staging {
+  struct Copy {
+    const double *src;
+    long ld;
+    int rows, cols;  // of the source block
+    long pitch;      // of the copy
+    size_t off;      // doubles into the workspace
+    bool transpose;
+  };
+  std::vector<Copy> copies;
+  std::vector<int> a_copy, b_copy;  // per descriptor; b_copy -1: B is read in place
+  size_t total = 0;                 // doubles
+  void plan(const GemmDesc *descs, int ndesc);
+  int run(double *ws, hipStream_t stream) const;
+  void apply(double *ws, GemmDesc *descs, int ndesc) const;
+};
 int phpc_f64_grouped_dev_at(const GemmDesc *descs_dev, const GemmDesc *descs_host, int ndesc, int grid,
                             int config_id, hipStream_t stream);
 // the planner's choice, with the transposed-A configuration for large single-panel work

@@ -180,3 +180,27 @@ def test_check_sampled_matches_oracle():
     C2[rows[3], cols[3]] += 1e-6
     cw2, _ = phpc.check_sampled(dA, dB, torch.from_numpy(C2).cuda(), rows, cols, C0=dC0)
     assert cw2 > 1e-9
+
+
+@pytest.mark.parametrize("cfg", [10, 11, 12])
+@pytest.mark.parametrize("m,k,n", [(257, 131, 199), (300, 200, 260), (33, 17, 1), (1, 64, 3)])
+def test_transposed_a_configs_on_ragged_shapes(cfg, m, k, n):
+    """The transposed-A configurations (A staged transposed; B staged to an even pitch when
+    its rows are not 16-byte aligned) are bit-exact on odd and tiny shapes, through the
+    device entry point on strided views whose B rows start at odd column offsets."""
+    import torch
+    A, B, C0 = O.uniform((m, k), 41), O.uniform((k, n), 42), O.uniform((m, n), 43)
+    want = O.kernel_emul(A, B, C0.copy())
+    # B as a view with an odd column offset and an odd row stride
+    Bbig = torch.zeros((k, n + 3), dtype=torch.float64, device="cuda")
+    Bbig[:, 1:1 + n] = torch.from_numpy(B).cuda()
+    dB = Bbig[:, 1:1 + n]
+    dA = torch.from_numpy(A).cuda()
+    dC = torch.from_numpy(C0.copy()).cuda()
+    phpc.set_launch_policy(0, cfg, 0)
+    try:
+        phpc.dgemm_device(dA, dB, dC)
+        torch.cuda.synchronize()
+    finally:
+        phpc.set_launch_policy(0, -1, 0)
+    assert np.array_equal(bits(dC.cpu().numpy()), bits(want))
