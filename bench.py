@@ -372,7 +372,7 @@ def serving_bench(phpc, n_req, seed=11, slots=3, batch_max=32, split_min=4096, l
             "poisson": {},
         }
         for policy, aging, tag in ((phpc.SCHED_FIFO, 0.0, "fifo"), (phpc.SCHED_SJF, 0.0, "sjf"),
-                                   (phpc.SCHED_SJF, 8.0, "sjf_aging8")):
+                                   (phpc.SCHED_SJF, 8.0, "sjf_aging8"), (phpc.SCHED_LANES, 0.0, "lanes")):
             srv.set_policy(policy, aging)
             out["poisson"][tag] = []
             for load in loads:

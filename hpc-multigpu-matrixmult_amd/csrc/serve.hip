@@ -121,16 +121,37 @@ struct phpc_server {
 // the one with the least estimated device time less its waiting time divided by the
 // aging factor (shortest first). Requests that fill the GPU alone start only while fewer
 // than big_limit of them run (each one beyond the first only shares the GPU); small
-// requests may start beside them.
+// requests may start beside them. LANES: a free place among the GPU-filling requests
+// goes to the shortest of them first; everything else runs in arrival order.
 std::deque<Req *>::iterator phpc_server::pick_locked() {
   const clk::time_point now = clk::now();
+  auto key_of = [&](const Req *q) {
+    double key = q->est;
+    if (aging > 0.0) key -= std::chrono::duration<double>(now - q->arrive).count() / aging;
+    return key;
+  };
   auto best = queue.end();
   double best_key = 0.0;
+  if (policy == PHPC_SCHED_LANES) {
+    auto first_light = queue.end();
+    for (auto it = queue.begin(); it != queue.end(); ++it) {
+      if (!(*it)->heavy) {
+        if (first_light == queue.end()) first_light = it;
+        continue;
+      }
+      if (big_running >= big_limit) continue;
+      const double key = key_of(*it);
+      if (best == queue.end() || key < best_key) {
+        best = it;
+        best_key = key;
+      }
+    }
+    return best != queue.end() ? best : first_light;
+  }
   for (auto it = queue.begin(); it != queue.end(); ++it) {
     if (big_running >= big_limit && (*it)->heavy) continue;
     if (policy == PHPC_SCHED_FIFO) return it;
-    double key = (*it)->est;
-    if (aging > 0.0) key -= std::chrono::duration<double>(now - (*it)->arrive).count() / aging;
+    const double key = key_of(*it);
     if (best == queue.end() || key < best_key) {
       best = it;
       best_key = key;
@@ -434,7 +455,8 @@ extern "C" int phpc_server_set_split_min(phpc_server_t srv, int dim) {
 }
 
 extern "C" int phpc_server_set_policy(phpc_server_t srv, int policy, double aging) {
-  if (!srv || (policy != PHPC_SCHED_FIFO && policy != PHPC_SCHED_SJF) || !(aging >= 0.0))
+  if (!srv || (policy != PHPC_SCHED_FIFO && policy != PHPC_SCHED_SJF && policy != PHPC_SCHED_LANES) ||
+      !(aging >= 0.0))
     return phpc_set_error(PHPC_ERR_ARG, "bad scheduling policy");
   std::lock_guard<std::mutex> lk(srv->mu);
   srv->policy = policy;

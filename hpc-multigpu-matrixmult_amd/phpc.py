@@ -405,6 +405,7 @@ ACC_FP64 = 0
 ACC_FP32 = 1
 SCHED_FIFO = 0
 SCHED_SJF = 1
+SCHED_LANES = 2
 
 
 class Server:

@@ -69,6 +69,10 @@ int phpc_server_set_split_min(phpc_server_t server, int dim);
 /* Least estimated device time first (estimate from the flop count), less waiting time
  * divided by `aging` so that large requests are not starved; aging = 0: no credit. */
 #define PHPC_SCHED_SJF 1
+/* Two lanes: while fewer than the big limit of GPU-filling requests run, the waiting one
+ * with the least estimated device time (less waiting time / aging) starts first; every
+ * other request starts in arrival order beside them. */
+#define PHPC_SCHED_LANES 2
 int phpc_server_set_policy(phpc_server_t server, int policy, double aging);
 /* Requests that fill the GPU alone (output tiles >= compute units) and whose estimated
  * device time is at least min_seconds: at most `limit` of them in flight at once

@@ -72,7 +72,7 @@ def out_array(acc, m, n):
     return np.full((m, n), np.nan, dtype=np.float64 if acc == phpc.ACC_FP64 else np.float32)
 
 
-@pytest.mark.parametrize("policy", [phpc.SCHED_FIFO, phpc.SCHED_SJF])
+@pytest.mark.parametrize("policy", [phpc.SCHED_FIFO, phpc.SCHED_SJF, phpc.SCHED_LANES])
 def test_alone_vs_mix_vs_one_shot(policy):
     srv = phpc.Server(8 * GiB, slots=3, max_dim=1024, batch_max=8)
     srv.set_split_min(SPLIT_MIN)
