@@ -1166,6 +1166,19 @@ void F64Staging::apply(double *ws, GemmDesc *descs, int ndesc) const {
   }
 }
 
+int phpc_f64_reserve_staging(const GemmDesc *descs, int ndesc, hipStream_t stream) {
+  const int cfg = phpc_f64_pick_config_at(descs, ndesc);
+  if (!kConfigs[cfg].trans_a) return PHPC_OK;
+  F64Staging st;
+  st.plan(descs, ndesc);
+  std::lock_guard<std::mutex> lk(g_at_ws.mu);
+  g_at_ws.after_previous(stream);
+  if (!g_at_ws.ws.get(st.total * sizeof(double), 0, stream))
+    return phpc_set_error(PHPC_ERR_MEM, "transposed-A workspace allocation failed (%zu bytes)", st.total * 8);
+  g_at_ws.mark_used(stream);
+  return PHPC_OK;
+}
+
 int phpc_f64_transpose(const double *A, long lda, int m, int k, double *At, long ldat, hipStream_t stream) {
   if (m <= 0 || k <= 0) return PHPC_OK;
   const dim3 tg((unsigned)((k + 31) / 32), (unsigned)((ldat + 31) / 32));

@@ -148,6 +148,9 @@ int phpc_f64_grouped_dev_at(const GemmDesc *descs_dev, const GemmDesc *descs_hos
 // the planner's choice, with the transposed-A configuration for large single-panel work
 // whose B allows it
 int phpc_f64_pick_config_at(const GemmDesc *descs, int ndesc);
+// Size the process-wide staging workspace for these descriptors now, so that a pipeline
+// of host-pointer launches does not allocate between its copies.
+int phpc_f64_reserve_staging(const GemmDesc *descs, int ndesc, hipStream_t stream);
 
 // Strided block pack: dst[r*ldd + c] = src[r*lds + c] for r < rows, c < cols.
 // Host <-> device copies through pinned staging buffers (rows x width bytes, strided).

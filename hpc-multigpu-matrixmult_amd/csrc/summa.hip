@@ -212,6 +212,19 @@ extern "C" int phpc_summa_host(int p, int q, int panel, int mode, const double *
     ev_e.emplace_back();
     if (!ev_in.back().e || !ev_s.back().e || !ev_e.back().e) return phpc_set_error(PHPC_ERR_HIP, "event creation failed");
   }
+  {
+    // the largest band's staging copies, allocated before the copies start
+    GemmDesc d{};
+    d.A = dA;
+    d.B = dB;
+    d.C = dC;
+    d.lda = d.ldb = d.ldc = N;
+    d.m = std::min(hb, N);
+    d.n = d.k = N;
+    d.panel = mode == PHPC_SUMMA_CHAINED ? 0 : w;
+    if (pol.config_id < 0)
+      if (int rc0 = phpc_f64_reserve_staging(&d, 1, sc)) return rc0;
+  }
   int rc = phpc_stage_h2d(dB, row, B, row, row, N, sx);
   for (int i = 0; i <= nb && rc == PHPC_OK; ++i) {
     if (i < nb) {

@@ -1,0 +1,110 @@
+"""The exact-arithmetic steps of the CRT split GEMM's residue and reconstruction kernels
+(csrc/split.hip), checked on the CPU with IEEE float32 arithmetic as the device does it:
+
+- residues: for every odd modulus and every byte-dot-product sum the kernels can form
+  (0 .. 9 * 255^2), q = rint(float32(sum) * float32(1/m)) is the exact nearest quotient,
+  so sum - q*m is the symmetric residue;
+- Garner digits: with the coefficients stored in the symmetric (int8) range, the digit sums
+  stay below 2^18, the same rint reduction is exact there, and the mixed-radix digits
+  rebuild every integer in the symmetric range of the moduli product; the plain-FP64
+  Horner prefix (top digits while their product is <= 2^53) is exact.
+"""
+import math
+
+import numpy as np
+
+MODULI = [256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193, 191, 181, 179, 173]
+
+
+def sym(x, m):
+    r = x % m
+    return r - m if r > (m - 1) // 2 else r
+
+
+def test_residue_rint_reduction_is_exact_over_the_whole_sum_range():
+    sums = np.arange(0, 9 * 255 * 255 + 1, dtype=np.int64)
+    sf = sums.astype(np.float32)
+    for m in MODULI[1:]:
+        assert m % 2 == 1
+        rmf = np.float32(1.0) / np.float32(m)
+        q = np.rint((sf * rmf).astype(np.float32)).astype(np.int64)
+        r = sums - q * m
+        assert np.all(np.abs(r) <= (m - 1) // 2), m
+        assert np.all((sums - r) % m == 0), m
+        # the device forms r with one FMA in float32: exact because |q m| < 2^24
+        assert int(np.max(np.abs(q * m))) < 2 ** 24
+
+
+def test_byte_weights_give_the_residue_of_the_integer():
+    rng = np.random.default_rng(3)
+    for m in MODULI[1:]:
+        w = [pow(2, 8 * i, m) for i in range(9)]
+        for _ in range(300):
+            x = int(rng.integers(0, 2 ** 62)) * int(rng.integers(1, 2 ** 10))
+            x %= 2 ** 72
+            b = [(x >> (8 * i)) & 0xFF for i in range(9)]
+            s = sum(bi * wi for bi, wi in zip(b, w))
+            assert s <= 9 * 255 * 255
+            assert sym(s, m) == sym(x, m)
+
+
+def garner_constants(L):
+    invs, q = [0] * L, [[0] * L for _ in range(L)]
+    for t in range(L):
+        m = MODULI[t]
+        prod, pmod = 1, []
+        for s in range(t):
+            pmod.append(prod)
+            prod = prod * MODULI[s] % m
+        inv = 1 if t == 0 else pow(prod, -1, m)
+        invs[t] = sym(inv, m)
+        for s in range(t):
+            q[s][t] = sym((m - pmod[s] * inv % m) % m, m)
+    return invs, q
+
+
+def plain_start(L):
+    prod, s = MODULI[L - 1], L - 1
+    while s > 0 and prod * MODULI[s - 1] <= 2 ** 53:
+        prod *= MODULI[s - 1]
+        s -= 1
+    return s
+
+
+def test_garner_digits_and_horner_rebuild_every_integer():
+    rng = np.random.default_rng(5)
+    for L in (8, 12, 13, 14, 15, 16, 20):
+        invs, q = garner_constants(L)
+        assert all(-128 <= c <= 127 for c in invs) and all(-128 <= c <= 127 for row in q for c in row)
+        M = math.prod(MODULI[:L])
+        S = plain_start(L)
+        assert math.prod(MODULI[S:L]) <= 2 ** 53
+        for trial in range(400):
+            x = int(rng.integers(-2 ** 62, 2 ** 62)) * int(rng.integers(1, 2 ** 40)) % M
+            x = x - M if x >= M // 2 else x
+            if trial == 0:
+                x = M // 2 - 1  # M is even (m_0 = 256): the symmetric range is [-M/2, M/2)
+            if trial == 1:
+                x = -(M // 2)
+            r = [sym(x, m) for m in MODULI[:L]]
+            v = [r[0]]
+            for t in range(1, L):
+                y = r[t] * invs[t] + sum(v[s] * q[s][t] for s in range(t))
+                assert abs(y) < 2 ** 18
+                yf = np.float32(y)
+                qq = np.rint(np.float32(yf * (np.float32(1.0) / np.float32(MODULI[t]))))
+                d = int(y - int(qq) * MODULI[t])
+                assert abs(d) <= (MODULI[t] - 1) // 2
+                v.append(d)
+            # plain FP64 Horner on the top digits is exact
+            hi = float(v[L - 1])
+            for t in range(L - 2, S - 1, -1):
+                hi = math.fma(hi, float(MODULI[t]), float(v[t])) if hasattr(math, "fma") else hi * MODULI[t] + v[t]
+            exact = v[L - 1]
+            for t in range(L - 2, S - 1, -1):
+                exact = exact * MODULI[t] + v[t]
+            assert hi == exact
+            val = exact
+            for t in range(S - 1, -1, -1):
+                val = val * MODULI[t] + v[t]
+            assert val == x, (L, x)
