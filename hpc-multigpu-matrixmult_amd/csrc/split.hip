@@ -511,9 +511,16 @@ __global__ __launch_bounds__(512) void i8_level_256x256_w8_k128_sd8(LevelArgs a)
   i8_level_body<256, 256, 2, 4, 128, 2, 8>(a);
 }
 __global__ __launch_bounds__(512) void i8_level_256x256_w8_sd8(LevelArgs a) { i8_level_body<256, 256, 2, 4, 64, 3, 8>(a); }
+// 16 waves (4 per SIMD) with 64x64 wave tiles: more waves to cover the slab loads
+__global__ __launch_bounds__(1024) void i8_level_256x256_w16_k128(LevelArgs a) {
+  i8_level_body<256, 256, 4, 4, 128, 2>(a);
+}
 // BF16 slices on the same pipeline (FP32-accurate 3 x BF16 GEMM)
 __global__ __launch_bounds__(512) void bf16_level_256x256_w8_k128(LevelArgs a) {
   i8_level_body<256, 256, 2, 4, 128, 2, 16, true>(a);
+}
+__global__ __launch_bounds__(1024) void bf16_level_256x256_w16_k128(LevelArgs a) {
+  i8_level_body<256, 256, 4, 4, 128, 2, 16, true>(a);
 }
 __global__ __launch_bounds__(256) void bf16_level_128x128_w4(LevelArgs a) {
   i8_level_body<128, 128, 2, 2, 64, 4, 16, true>(a);
@@ -530,21 +537,23 @@ struct I8Config {
 const I8Config kI8Configs[] = {
     I8CFG(256, 256, 2, 4, 64, 3, i8_level_256x256_w8),            // 0: 8 waves (wave tile 128x64), 3 stages
     I8CFG(128, 128, 2, 2, 64, 4, i8_level_128x128_w4),            // 1: 4 waves, 64x64 wave tiles (small problems)
-    I8CFG(256, 256, 2, 4, 128, 2, i8_level_256x256_w8_k128),      // 2: 8 waves, 128-byte slabs (default)
+    I8CFG(256, 256, 2, 4, 128, 2, i8_level_256x256_w8_k128),      // 2: 8 waves, 128-byte slabs
     I8CFG(256, 128, 4, 2, 64, 4, i8_level_256x128_w8),            // 3: 8 waves, wave tile 64x64
     I8CFG(256, 256, 2, 4, 64, 4, i8_level_256x256_w8_s4),         // 4: 8 waves, 4-stage pipeline
     I8CFG(256, 256, 2, 4, 64, 5, i8_level_256x256_w8_s5),         // 5: 8 waves, 5-stage pipeline
     I8CFG(256, 256, 2, 4, 128, 2, i8_level_256x256_w8_k128_sd8),  // 6: as 2, 32-bank swizzle
     I8CFG(256, 256, 2, 4, 64, 3, i8_level_256x256_w8_sd8),        // 7: as 0, 32-bank swizzle
+    I8CFG(256, 256, 4, 4, 128, 2, i8_level_256x256_w16_k128),     // 8: 16 waves, 64x64 wave tiles (default)
 };
 constexpr int kNumI8 = sizeof(kI8Configs) / sizeof(kI8Configs[0]);
-int g_i8_big = 2;  // kI8Configs index for large problems
+int g_i8_big = 8;  // kI8Configs index for large problems
 
 // BF16-slice level kernels (FP32-accurate 3 x BF16 GEMM), kept apart from the int8
 // table so that phpc_split_set_config can only pick int8 kernels
 const I8Config kBfConfigs[] = {
-    I8CFG(256, 256, 2, 4, 128, 2, bf16_level_256x256_w8_k128),  // 0: large problems
-    I8CFG(128, 128, 2, 2, 64, 4, bf16_level_128x128_w4),        // 1: small problems
+    I8CFG(256, 256, 4, 4, 128, 2, bf16_level_256x256_w16_k128),  // 0: large problems, 16 waves
+    I8CFG(128, 128, 2, 2, 64, 4, bf16_level_128x128_w4),         // 1: small problems
+    I8CFG(256, 256, 2, 4, 128, 2, bf16_level_256x256_w8_k128),   // 2: 8 waves
 };
 constexpr int kNumBf = sizeof(kBfConfigs) / sizeof(kBfConfigs[0]);
 
