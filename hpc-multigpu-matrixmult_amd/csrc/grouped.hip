@@ -251,6 +251,13 @@ extern "C" int phpc_plan_create(int dtype, const phpc_gemm_item *items, int coun
     phpc_plan_destroy(p);
     return rc;
   }
+  if (p->at_ws && !(dtype == PHPC_F64 && phpc_f64_config_trans_a(cfg))) {
+    // the autotuner tried a transposed-A configuration but kept another one
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(p->at_ws);
+    p->at_ws = nullptr;
+    p->d64_at.clear();
+  }
   p->grid = 0;
   *out = p;
   return phpc_set_error(PHPC_OK, nullptr);
