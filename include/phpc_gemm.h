@@ -85,8 +85,12 @@ int phpc_dgemm_vendor_device(const double *dA, long lda, const double *dB, long 
  */
 int phpc_set_launch_policy(int honor_grid, int config_id, int grid);
 int phpc_num_configs(void);
-/* Tuning knobs used by the autotuner: key 0 = FP64 config for large problems,
- * key 1 = tile-row swizzle width (1..64), key 2 = XCD-contiguous tile order (0/1). */
+/* Tuning knobs used by the autotuner: key 0 = FP64 config for large panel-grouped
+ * problems on device-resident descriptor lists (plans fall back to it), key 1 = tile-row
+ * swizzle width (1..64), key 2 = XCD-contiguous tile order (0/1), key 3 = FP64 config for
+ * large single-panel problems on device-resident descriptor lists, key 4 / key 5 = FP64
+ * config for large single-panel / panel-grouped problems through the entry points that
+ * may stage a transposed copy of A (configs 10-12 read such a copy). */
 int phpc_set_tuning(int key, int value);
 /* Describe configuration `id`: macro tile bm x bn, threads per workgroup. */
 int phpc_config_info(int id, int *bm, int *bn, int *threads);
