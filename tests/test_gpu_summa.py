@@ -243,3 +243,28 @@ def test_gemm_cuda_staged_copies_span_chunks():
     torch.cuda.synchronize()
     assert np.array_equal(bits(Ch[:, :n]), bits(dC.cpu().numpy()))
     assert np.array_equal(bits(Ch[:, n:]), bits(Cb[:, n:]))  # padding columns untouched
+
+
+def test_chained_transposed_a_matches_register_staged_full_matrix():
+    """N = 8192, 4 x 4 grid, CHAINED: the transposed-A kernel (default for large problems)
+    and the register-staged 256x128 kernel (configuration 6) give the same C bit for bit
+    over the whole matrix, on uniform and wide-exponent inputs."""
+    import torch
+    N = 8192
+    dA = torch.empty((N, N), dtype=torch.float64, device="cuda")
+    dB = torch.empty_like(dA)
+    for dist in (phpc.DIST_UNIFORM, phpc.DIST_WIDE):
+        print(f"chained at vs staged dist={dist}", flush=True)
+        phpc.fill_device(dA, dist, 31)
+        phpc.fill_device(dB, dist, 32)
+        dC1 = torch.zeros_like(dA)
+        phpc.summa_device(dA, dB, dC1, 4, 4, panel=256, mode=phpc.SUMMA_CHAINED)
+        dC2 = torch.zeros_like(dA)
+        phpc.set_launch_policy(0, 6, 0)
+        try:
+            phpc.summa_device(dA, dB, dC2, 4, 4, panel=256, mode=phpc.SUMMA_CHAINED)
+            torch.cuda.synchronize()
+        finally:
+            phpc.set_launch_policy(0, -1, 0)
+        assert torch.equal(dC1.view(torch.int64), dC2.view(torch.int64)), dist
+        del dC1, dC2

@@ -30,3 +30,24 @@ def test_reference_sweeps_listed():
     assert set(sweep.SWEEPS) == {"test0", "testA1", "testA2", "testB", "testC", "testD"}
     runs, key = sweep.SWEEPS["testA2"]
     assert (4, 2048, 32, 8, 8) in runs and key == "threads"
+
+
+def test_gathered_csv_reads_like_the_reference_plots(tmp_path):
+    """scripts/plots.py:183-208 reads a gathered sweep file with pandas and uses the
+    columns matrix_size, n_proc, total_threads, time_cuda, time_cuda_gpu and time_cublas;
+    the file sweep.py writes (with the process() columns) serves those reads unchanged."""
+    import pandas as pd
+    rows = ["1024,4,1,1,1024,1024,0.010000,0.004000,0.020000",
+            "2048,16,1,4,1024,4096,0.040000,0.016000,0.050000"]
+    lines = sweep.process(rows, {1024: 1.5, 2048: 12.0})
+    path = tmp_path / "testB.csv"
+    path.write_text("\n".join(lines) + "\n")
+    results = pd.read_csv(path)
+    total_threads = results["n_proc"] * results["total_threads"]
+    assert list(total_threads) == [4096, 65536]
+    assert list(results["matrix_size"]) == [1024, 2048]
+    t1 = 1.5
+    speedup = t1 / results["time_cuda"]
+    assert abs(speedup[0] - 150.0) < 1e-9
+    assert abs(results["time_cuda_gpu"][1] - 0.016) < 1e-12 and abs(results["time_cublas"][1] - 0.05) < 1e-12
+    assert abs(results["speedup"][0] - 150.0) < 1e-9 and abs(results["efficiency_kernel"][1] - 0.183105) < 1e-9
