@@ -387,6 +387,171 @@ __global__ __launch_bounds__(256) void f32x3_gemm_kernel(F32Batch batch) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Transposed-A direct-to-LDS FP32 GEMM for large products: the caller stores A
+// transposed (At[k][m], row pitch = the descriptor's lda, a multiple of 4 floats, pad
+// columns zero) and B with 16-byte aligned rows, so both 16-row slabs are k-row-major and
+// reach LDS by 16-byte buffer loads with no staging registers and no LDS write
+// instructions. A 1 KiB load instruction fills two 128-float k rows; within a row the
+// 16-byte chunks of odd rows are XOR-permuted by 8, so the two half-waves of an MFMA
+// operand read (rows k and k+1) hit opposite halves of the banks. Two stages.
+// ---------------------------------------------------------------------------
+typedef __attribute__((address_space(3))) void lds_void;
+
+template <int N>
+__device__ __forceinline__ void f32_wait_vm() {
+  __builtin_amdgcn_s_waitcnt(0x0F70 | (N & 0xF) | (((N >> 4) & 3) << 14));
+}
+
+__device__ __forceinline__ void f32_at_body(const F32Batch &batch) {
+  constexpr int BM = 128, BN = 128, WTM = 64, WTN = 64, TM = 2, TN = 2;
+  constexpr int ROW_BYTES = 512;                  // one 128-float k row
+  constexpr int A_BYTES = BK * ROW_BYTES;         // 16 rows
+  constexpr int STAGE_BYTES = 2 * A_BYTES;        // A then B
+  constexpr int A_BLOCKS = A_BYTES / 1024, BLOCKS = 2 * A_BLOCKS, PER = BLOCKS / 4;
+  extern __shared__ __attribute__((aligned(1024))) char smem[];
+  const int tid = threadIdx.x;
+  const int wave = uni(tid >> 6);
+  const int lane = tid & 63;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int G = gridDim.x, bid = blockIdx.x;
+  const int slot = uni((G % 8) == 0 ? (bid % 8) * (G / 8) + bid / 8 : bid);
+  // this lane's place in any 1 KiB block: row pair offset (lane / 32) and chunk
+  const int lrow = lane >> 5;
+  const int lchunk = (lane & 31) ^ (lrow << 3);
+
+  for (int round = 0;; ++round) {
+    const int t = round * G + slot;
+    if (t >= batch.total_tiles) break;
+    const F32Desc d = uniform_desc(batch, uni(t));
+    const int lt = t - d.tile_start;
+    const int row0 = uni((lt / d.tiles_n) * BM);
+    const int col0 = uni((lt % d.tiles_n) * BN);
+    const int m_rem = d.m - row0, n_rem = d.n - col0;
+    const long ldat = d.lda, ldb = d.ldb, ldc = d.ldc;
+    const float *At_g = d.A + row0;
+    const float *Bg = d.B + col0;
+    gfloat *Cg = (gfloat *)(d.C + (long)row0 * ldc + col0);
+    const int Kdim = d.k;
+    const int nslab = (Kdim + BK - 1) / BK;
+
+    uint32_t goff[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int blk = wave + u * 4;
+      const bool isA = blk < A_BLOCKS;
+      const int r = 2 * (isA ? blk : blk - A_BLOCKS) + lrow;  // k row within the slab
+      const int c = lchunk * 4;                               // column of the chunk
+      const int rem = isA ? m_rem : n_rem;
+      goff[u] = c < rem ? (uint32_t)(((long)r * (isA ? ldat : ldb) + c) * 4) : kOOB;
+    }
+    const int full_a = (int)(BK * ldat * 4), full_b = (int)(BK * ldb * 4);
+    auto issue = [&](int sidx) {
+      const int kb = sidx * BK;
+      const int krem = min(BK, Kdim - kb);
+      char *stage = smem + (sidx & 1) * STAGE_BYTES;
+      const int na = krem == BK ? full_a : krem * (int)ldat * 4;
+      const int nb = krem == BK ? full_b : krem * (int)ldb * 4;
+      const rsrc_t rA = __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(At_g + (long)kb * ldat), (short)0, na,
+                                                          0x00020000);
+      const rsrc_t rB =
+          __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(Bg + (long)kb * ldb), (short)0, nb, 0x00020000);
+#pragma unroll
+      for (int u = 0; u < PER; ++u) {
+        const int blk = wave + u * 4;
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(blk < A_BLOCKS ? rA : rB, (lds_void *)(stage + blk * 1024), 16,
+                                                  goff[u], 0, 0, 0);
+      }
+    };
+
+    f16v acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) acc[i][j] = f16v{};
+
+    // LDS read offsets of this lane's operands for k step kk: row k = 2 kk + lane / 32
+    int a_off[TM], b_off[TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      const int col = wm * WTM + i * 32 + (lane & 31);
+      a_off[i] = lrow * ROW_BYTES + (((col >> 2) ^ (lrow << 3)) << 4) + (col & 3) * 4;
+    }
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int col = wn * WTN + j * 32 + (lane & 31);
+      b_off[j] = A_BYTES + lrow * ROW_BYTES + (((col >> 2) ^ (lrow << 3)) << 4) + (col & 3) * 4;
+    }
+
+    __syncthreads();
+    if (nslab > 0) issue(0);
+    for (int sidx = 0; sidx < nslab; ++sidx) {
+      f32_wait_vm<0>();
+      __syncthreads();
+      const char *stage = smem + (sidx & 1) * STAGE_BYTES;
+      auto kstep = [&](int kk) {
+        float a[TM], b[TN];
+#pragma unroll
+        for (int i = 0; i < TM; ++i) a[i] = *reinterpret_cast<const float *>(stage + a_off[i] + kk * 2 * ROW_BYTES);
+#pragma unroll
+        for (int j = 0; j < TN; ++j) b[j] = *reinterpret_cast<const float *>(stage + b_off[j] + kk * 2 * ROW_BYTES);
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i], b[j], acc[i][j], 0, 0, 0);
+      };
+      kstep(0);
+      if (sidx + 1 < nslab) issue(sidx + 1);
+#pragma unroll
+      for (int kk = 1; kk < BK / 2; ++kk) kstep(kk);
+    }
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const int cc = wn * WTN + j * 32 + (lane & 31);
+        if (cc >= n_rem) continue;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int rr = wm * WTM + i * 32 + 8 * (r / 4) + 4 * (lane >> 5) + (r % 4);
+          if (rr < m_rem) {
+            gfloat *pc = Cg + (long)rr * ldc + cc;
+            *pc = *pc + acc[i][j][r];
+          }
+        }
+      }
+  }
+}
+
+__global__ __launch_bounds__(256) void f32_at_128x128(F32Batch batch) { f32_at_body(batch); }
+
+// At[k][m] = A[m][k] for m < M, k < K; 0 for M <= m < ldat
+__global__ __launch_bounds__(256) void f32_transpose_kernel(const float *__restrict__ A, long lda, int M, int Kd,
+                                                            float *__restrict__ At, long ldat) {
+  __shared__ float tile[32][33];
+  const int k0 = blockIdx.x * 32, m0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+#pragma unroll
+  for (int r = ty; r < 32; r += 8) {
+    const int m = m0 + r, k = k0 + tx;
+    tile[r][tx] = (m < M && k < Kd) ? A[(long)m * lda + k] : 0.0f;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = ty; r < 32; r += 8) {
+    const int k = k0 + r, m = m0 + tx;
+    if (k < Kd && m < ldat) At[(long)k * ldat + m] = tile[tx][r];
+  }
+}
+
+// Bc[k][j] = B[k][j] for j < N, 0 for N <= j < ldbc
+__global__ __launch_bounds__(256) void f32_copy_pad_kernel(const float *__restrict__ B, long ldb, int Kd, int N,
+                                                           float *__restrict__ Bc, long ldbc) {
+  const int k = blockIdx.y;
+  for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < ldbc; j += (long)gridDim.x * blockDim.x)
+    Bc[(long)k * ldbc + j] = j < N ? B[(long)k * ldb + j] : 0.0f;
+}
+
 typedef void (*F32Fn)(F32Batch);
 struct F32Cfg {
   int bm, bn;
@@ -421,6 +586,66 @@ int occ(int c, int v) {
 }
 
 bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+int g_f32_at = 1;  // large single FP32 products through the transposed-A kernel
+SharedWorkspace g_f32_at_ws;
+int g_f32_at_occ = 0;
+
+// One large FP32 GEMM: A staged transposed (and B to a pitch of 4 floats when its rows are
+// not 16-byte aligned), then the transposed-A kernel over 128 x 128 tiles. With an aligned B
+// and a pitch of 4 floats, the 16-byte chunk holding a row's last column starts on a 16-byte
+// boundary and so lies in the same page as that column; its extra columns only feed output
+// columns that are never stored.
+int f32_launch_transposed(F32Desc d, hipStream_t stream) {
+  const long ldat = ((long)d.m + 3) / 4 * 4;
+  const bool copy_b = !aligned16(d.B) || d.ldb % 4;
+  const long ldbc = ((long)d.n + 3) / 4 * 4;
+  const size_t a_elems = ((size_t)d.k * ldat + 63) / 64 * 64;
+  const size_t total = a_elems + (copy_b ? (size_t)d.k * ldbc : 0);
+  if (16L * ldat * 4 >= 0x7fffffffL || 16L * (copy_b ? ldbc : d.ldb) * 4 >= 0x7fffffffL ||
+      128L * d.ldc * 4 >= 0x7fffffffL)
+    return phpc_set_error(PHPC_ERR_ARG, "leading dimension too large");
+  std::lock_guard<std::mutex> lk(g_f32_at_ws.mu);
+  g_f32_at_ws.after_previous(stream);
+  float *ws = static_cast<float *>(g_f32_at_ws.ws.get(total * sizeof(float), 0, stream));
+  if (!ws) return phpc_set_error(PHPC_ERR_MEM, "FP32 staging workspace allocation failed");
+  hipLaunchKernelGGL(f32_transpose_kernel, dim3((unsigned)((d.k + 31) / 32), (unsigned)((ldat + 31) / 32)), dim3(256),
+                     0, stream, d.A, d.lda, d.m, d.k, ws, ldat);
+  PHPC_HIP(hipGetLastError());
+  if (copy_b) {
+    hipLaunchKernelGGL(f32_copy_pad_kernel, dim3((unsigned)std::min<long>((ldbc + 255) / 256, 64), (unsigned)d.k),
+                       dim3(256), 0, stream, d.B, d.ldb, d.k, d.n, ws + a_elems, ldbc);
+    PHPC_HIP(hipGetLastError());
+    d.B = ws + a_elems;
+    d.ldb = ldbc;
+  }
+  d.A = ws;
+  d.lda = ldat;
+  d.tiles_n = (d.n + 127) / 128;
+  d.tile_start = 0;
+  const int total_tiles = ((d.m + 127) / 128) * d.tiles_n;
+  constexpr size_t lds = 2 * 2 * 16 * 512;
+  if (!g_f32_at_occ) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(f32_at_128x128),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void *>(f32_at_128x128), 256, lds) !=
+            hipSuccess ||
+        n < 1)
+      n = 1;
+    g_f32_at_occ = n;
+  }
+  F32Batch b{};
+  b.ndesc = 1;
+  b.total_tiles = total_tiles;
+  b.d0 = d;
+  int G = g_f32_at_occ * phpc_device().cus;
+  if (G > total_tiles) G = total_tiles;
+  hipLaunchKernelGGL(f32_at_128x128, dim3(G), dim3(256), lds, stream, b);
+  PHPC_HIP(hipGetLastError());
+  g_f32_at_ws.mark_used(stream);
+  return PHPC_OK;
+}
 
 }  // namespace
 
@@ -546,8 +771,14 @@ extern "C" int phpc_sgemm_device(const float *dA, long lda, const float *dB, lon
   } else {
     const long work = (long)((m + 127) / 128) * ((n + 127) / 128);
     const int cfg = work >= 2L * phpc_device().cus ? 2 : 0;
-    const int tiles = phpc_f32_plan_tiles(&d, 1, cfg);
-    rc = phpc_f32_launch(nullptr, &d, 1, tiles, cfg, vec4, 0, (hipStream_t)stream);
+    if (cfg == 2 && g_f32_at && (m + 3L) / 4 * 4 / 32 < 65535 && k < 65535)
+      rc = f32_launch_transposed(d, (hipStream_t)stream);
+    else {
+      const int tiles = phpc_f32_plan_tiles(&d, 1, cfg);
+      rc = phpc_f32_launch(nullptr, &d, 1, tiles, cfg, vec4, 0, (hipStream_t)stream);
+    }
   }
   return rc ? rc : phpc_set_error(PHPC_OK, nullptr);
 }
+
+void phpc_f32_set_transposed_a(int on) { g_f32_at = on ? 1 : 0; }

@@ -1302,6 +1302,9 @@ extern "C" int phpc_set_tuning(int key, int value) {
         return phpc_set_error(PHPC_ERR_ARG, "config %d out of range", value);
       g_big_config_host = value;
       break;
+    case 6:
+      phpc_f32_set_transposed_a(value);
+      break;
     default:
       return phpc_set_error(PHPC_ERR_ARG, "unknown tuning key %d", key);
   }

@@ -68,6 +68,7 @@ struct F32Desc {
 int phpc_f32_num_configs();
 void phpc_f32_config_tile(int id, int *bm, int *bn);
 int phpc_f32_plan_tiles(F32Desc *descs, int ndesc, int id);
+void phpc_f32_set_transposed_a(int on);
 bool phpc_f32_vec4_ok(const F32Desc *descs, int ndesc);
 int phpc_f32x3_launch(const F32Desc *descs_dev, const F32Desc *descs_host, int ndesc, int total_tiles, bool vec4,
                       int grid, hipStream_t stream);

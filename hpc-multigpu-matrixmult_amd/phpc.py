@@ -327,6 +327,15 @@ def num_configs():
     return lib().phpc_num_configs()
 
 
+TUNE_F32_TRANSPOSED_A = 6
+
+
+def set_tuning(key, value):
+    """One tuning knob (phpc_set_tuning): keys 0-5 pick FP64 configs, key 6 switches large
+    single FP32 products between the transposed-A kernel (1) and the register-staged one (0)."""
+    check(lib().phpc_set_tuning(key, value))
+
+
 SPLIT_SLICES = 0
 SPLIT_CRT = 1
 

@@ -90,7 +90,9 @@ int phpc_num_configs(void);
  * swizzle width (1..64), key 2 = XCD-contiguous tile order (0/1), key 3 = FP64 config for
  * large single-panel problems on device-resident descriptor lists, key 4 / key 5 = FP64
  * config for large single-panel / panel-grouped problems through the entry points that
- * may stage a transposed copy of A (configs 10-12 read such a copy). */
+ * may stage a transposed copy of A (configs 10-12 read such a copy), key 6 = large
+ * single FP32 products through phpc_sgemm_device on the transposed-A FP32 kernel (1,
+ * default) or on the register-staged one (0). */
 int phpc_set_tuning(int key, int value);
 /* Describe configuration `id`: macro tile bm x bn, threads per workgroup. */
 int phpc_config_info(int id, int *bm, int *bn, int *threads);

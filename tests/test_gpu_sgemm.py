@@ -35,6 +35,60 @@ def test_sgemm_within_fp32_bound(method, m, n, k):
     assert np.all(err <= bound(A, B, C0)), (m, n, k, float(np.max(err / bound(A, B, C0))))
 
 
+def _sgemm(A, B, C0, key6, b_view=None):
+    import torch
+    dB = torch.from_numpy(B).cuda() if b_view is None else b_view
+    dC = torch.from_numpy(C0.copy()).cuda()
+    phpc.set_tuning(phpc.TUNE_F32_TRANSPOSED_A, key6)
+    try:
+        phpc.sgemm_device(torch.from_numpy(A).cuda(), dB, dC, method=phpc.SGEMM_FP32)
+    finally:
+        phpc.set_tuning(phpc.TUNE_F32_TRANSPOSED_A, 1)
+    torch.cuda.synchronize()
+    return dC.cpu().numpy()
+
+
+# large enough (>= 2 tiles of 128 x 128 per CU) for the transposed-A kernel; n = 2052 keeps
+# B as it is (16-byte rows), n = 2049 / 2051 stage it to a pitch of 4 floats
+@pytest.mark.parametrize("m,n,k", [(4100, 2052, 300), (4097, 2049, 1000), (4096, 2051, 17), (4224, 4096, 2)])
+def test_fp32_transposed_a_bit_identical_to_register_staged(m, n, k):
+    """The transposed-A FP32 kernel takes k in the same pairs, in the same order, as the
+    register-staged kernel, so the two agree bit for bit; both stay within the FP32 bound."""
+    A = O.uniform((m, k), 11).astype(np.float32)
+    B = O.uniform((k, n), 12).astype(np.float32)
+    C0 = O.uniform((m, n), 13).astype(np.float32)
+    got = _sgemm(A, B, C0, 1)
+    want = _sgemm(A, B, C0, 0)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (m, n, k)
+    ref = C0.astype(np.float64) + A.astype(np.float64) @ B.astype(np.float64)
+    assert np.all(np.abs(got.astype(np.float64) - ref) <= bound(A, B, C0))
+
+
+def test_fp32_transposed_a_on_offset_views():
+    """B a column-offset view (base not 16-byte aligned) and A with a padded pitch."""
+    import torch
+    m, n, k = 4160, 2080, 257
+    Ap = O.uniform((m, k + 3), 21).astype(np.float32)
+    Bp = O.uniform((k, n + 5), 22).astype(np.float32)
+    C0 = O.uniform((m, n), 23).astype(np.float32)
+    dA = torch.from_numpy(Ap).cuda()[:, 1:k + 1]
+    dB = torch.from_numpy(Bp).cuda()[:, 1:n + 1]
+    outs = []
+    for key6 in (1, 0):
+        dC = torch.from_numpy(C0.copy()).cuda()
+        phpc.set_tuning(phpc.TUNE_F32_TRANSPOSED_A, key6)
+        try:
+            phpc.sgemm_device(dA, dB, dC, method=phpc.SGEMM_FP32)
+        finally:
+            phpc.set_tuning(phpc.TUNE_F32_TRANSPOSED_A, 1)
+        torch.cuda.synchronize()
+        outs.append(dC.cpu().numpy())
+    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+    A, B = Ap[:, 1:k + 1], Bp[:, 1:n + 1]
+    ref = C0.astype(np.float64) + A.astype(np.float64) @ B.astype(np.float64)
+    assert np.all(np.abs(outs[0].astype(np.float64) - ref) <= bound(A, B, C0))
+
+
 def test_bf16x3_wide_range_inputs():
     """Entries spread over 2^-40 .. 2^40: each slice keeps its own exponent."""
     import torch
