@@ -694,8 +694,9 @@ double split_error_estimate(int s, double ratio_a, double ratio_b, int k) {
 // A' = rint(A[i,l] 2^(p - e_i)) and B' = rint(B[l,j] 2^(p - f_j)) are integers of at most
 // p+1 bits; C' = A' B' is computed exactly from its residues modulo L pairwise coprime
 // moduli m_t <= 256 (residues fit int8), one int8 GEMM per modulus, and
-// C += 2^(e_i + f_j - 2p) C'. The moduli product M must exceed 2 max|C'|, i.e.
-// sum log2 m_t >= 2p + ceil(log2 k) + 2.
+// C += 2^(e_i + f_j - 2p) C'. The moduli product M must exceed 2 max|C'|: with the
+// caller's modulus count, sum log2 m_t >= 2p + ceil(log2 k) + 2 (any inputs); with the
+// automatic count, the bound from the row and column ratios (crt_needed_bits).
 constexpr int kMaxMods = 20;
 constexpr int kModuli[kMaxMods] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227,
                                    223, 217, 211, 199, 197, 193, 191, 181, 179, 173};
@@ -1002,6 +1003,27 @@ double crt_bits(int L) {
   return b;
 }
 
+// Bits the moduli product needs for C' = A' B' at integer width p. |A'_il| <= |a_il|
+// 2^(p - e_i) + 1/2, so sum_l |A'_il| <= 2^p k / ratio_i + k / 2, and with |B'_lj| <= 2^p,
+// |C'_ij| <= 2^p k (2^p / ratio_i + 1/2); the same holds with the column ratio of B. R is
+// the larger of min_i ratio_i and min_j ratio_j over nonzero rows and columns (zero ones
+// give C' = 0), R >= 1 since 2^e_i exceeds every |a_il|; it is shrunk by 1e-6 to cover the
+// rounding of the FP64 row sums (k u < 2.4e-7 for k < 2^31). M > 2 max|C'| then holds with
+// one bit to spare when sum log2 m_t >= the returned value. For R = 1 this is the
+// worst-case 2p + log2 k + 2.
+double crt_needed_bits(int p, double R, int k) {
+  const double r = std::max(R * (1.0 - 1e-6), 1.0);
+  return p + std::log2((double)std::max(k, 1)) + std::log2(std::ldexp(1.0, p) / r + 0.5) + 2.0;
+}
+
+// smallest positive entry (1 when there is none)
+double min_positive(const std::vector<double> &v) {
+  double m = 0.0;
+  for (double x : v)
+    if (x > 0.0 && (m == 0.0 || x < m)) m = x;
+  return m > 0.0 ? m : 1.0;
+}
+
 // a-priori estimate for integer operands of p bits: each entry moves by at most
 // 2^(e - p - 1); the errors add like a random walk over k. The factor 2 covers the tail
 // of the maximum over all m*n outputs (the random-walk figure matches the measured
@@ -1114,11 +1136,12 @@ int phpc_dgemm_split_ws(const double *dA, long lda, const double *dB, long ldb, 
       pbits = (int)std::floor((crt_bits(L) - logk - 2) / 2);
     } else {
       const double target = target_err > 0 ? target_err : 1e-14;
+      const double R = std::max(min_positive(ra), min_positive(rb));
       for (pbits = 8; pbits < 70; ++pbits)
         if (crt_error_estimate(pbits, mra, mrb, k) <= target) break;
       for (L = 2; L < kMaxMods; ++L)
-        if (crt_bits(L) >= 2.0 * pbits + logk + 2) break;
-      if (crt_bits(L) < 2.0 * pbits + logk + 2) pbits = (int)std::floor((crt_bits(L) - logk - 2) / 2);
+        if (crt_bits(L) >= crt_needed_bits(pbits, R, k)) break;
+      while (pbits > 0 && crt_bits(L) < crt_needed_bits(pbits, R, k)) --pbits;
     }
     if (pbits < 1) return phpc_set_error(PHPC_ERR_ARG, "k = %d too large for the CRT scheme", k);
     const CrtConsts cc = make_crt(L);

@@ -119,6 +119,49 @@ def test_crt_matches_restatement_bitwise(m, k, n, L, gen):
     assert np.array_equal(bits(dC.cpu().numpy()), bits(want))
 
 
+MODULI = [256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193, 191, 181, 179, 173]
+
+
+def _max_abs_product_bits(A, B, p):
+    """log2 of max_ij sum_l |A'_il| |B'_lj| for the scaled integers A' = rint(a 2^(p - e_i)),
+    B' = rint(b 2^(p - f_j)), 2^e the power of two just above a row's (column's) largest |entry|."""
+    ea = np.frexp(np.abs(A).max(axis=1))[1]
+    eb = np.frexp(np.abs(B).max(axis=0))[1]
+    Ai = np.rint(np.ldexp(A, (p - ea)[:, None]))
+    Bi = np.rint(np.ldexp(B, (p - eb)[None, :]))
+    return float(np.log2((np.abs(Ai) @ np.abs(Bi)).max()))
+
+
+def test_crt_auto_moduli_cover_the_largest_product():
+    """The automatic modulus count must leave M = prod m_t above 2 max|C'| for the scaled
+    integers actually formed. All entries of one magnitude and sign (row and column ratios of
+    about 1) are its worst case, max|C'| ~ 2^(2p) k, and must come back exact; entries spread
+    over 2^-20 .. 2^20 get by with fewer bits than the worst case 2p + log2 k + 2."""
+    import torch
+    m, k, n = 64, 3000, 48
+    A = np.full((m, k), 1023.0)
+    B = np.full((k, n), 1021.0)
+    dC = torch.zeros((m, n), dtype=torch.float64, device="cuda")
+    info = phpc.dgemm_split_device(dev(A), dev(B), dC, slices=0, target=1e-14, method=phpc.SPLIT_CRT)
+    torch.cuda.synchronize()
+    assert np.array_equal(dC.cpu().numpy(), A @ B)
+    cum = np.cumsum(np.log2(MODULI))
+    assert cum[info.slices - 1] > 1 + _max_abs_product_bits(A, B, info.int_bits)
+    Aw, Bw, C0 = O.wide((m, k), 5), O.wide((k, n), 6), O.uniform((m, n), 7)
+    dC = dev(C0)
+    infw = phpc.dgemm_split_device(dev(Aw), dev(Bw), dC, slices=0, target=1e-14, method=phpc.SPLIT_CRT)
+    torch.cuda.synchronize()
+    need = 1 + _max_abs_product_bits(Aw, Bw, infw.int_bits)
+    worst = 2 * infw.int_bits + np.log2(k) + 2
+    print(f"  wide: L={infw.slices} p={infw.int_bits} bits={cum[infw.slices - 1]:.1f} need={need:.1f} "
+          f"worst={worst:.1f}", flush=True)
+    assert cum[infw.slices - 1] > need
+    assert need < worst - 4
+    # the same moduli and integer width through the restatement give the same bits
+    want = O.crt_gemm(Aw, Bw, C0.copy(), infw.int_bits, infw.slices)
+    assert np.array_equal(bits(dC.cpu().numpy()), bits(want))
+
+
 def test_crt_exact_on_integers():
     """Integer inputs whose product fits the moduli come back exact."""
     import torch
