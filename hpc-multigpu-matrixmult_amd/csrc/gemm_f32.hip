@@ -552,6 +552,46 @@ __global__ __launch_bounds__(256) void f32_copy_pad_kernel(const float *__restri
     Bc[(long)k * ldbc + j] = j < N ? B[(long)k * ldb + j] : 0.0f;
 }
 
+// All operand copies of a grouped transposed-A launch in one kernel: one workgroup per
+// 32 x 32 tile of a copy's destination, the copy found by binary search over block_start.
+__global__ __launch_bounds__(256) void f32_stage_kernel(const F32StageCopy *__restrict__ copies, int ncopies) {
+  __shared__ float tile[32][33];
+  const int b = blockIdx.x;
+  int lo = 0, hi = ncopies - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (copies[mid].block_start <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const F32StageCopy c = copies[lo];
+  const int lb = b - c.block_start;
+  const int d0 = (lb / c.tiles_x) * 32;  // first destination row of the tile
+  const int e0 = (lb % c.tiles_x) * 32;  // first destination column
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  if (c.transpose) {
+    // destination rows are source columns (k), destination columns source rows (m)
+#pragma unroll
+    for (int r = ty; r < 32; r += 8) {
+      const int sr = e0 + r, sc = d0 + tx;
+      tile[r][tx] = (sr < c.rows && sc < c.cols) ? c.src[(long)sr * c.ld + sc] : 0.0f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = ty; r < 32; r += 8) {
+      const int dr = d0 + r, dc = e0 + tx;
+      if (dr < c.cols && dc < c.pitch) c.dst[(long)dr * c.pitch + dc] = tile[tx][r];
+    }
+  } else {
+#pragma unroll
+    for (int r = ty; r < 32; r += 8) {
+      const int dr = d0 + r, dc = e0 + tx;
+      if (dr < c.rows && dc < c.pitch) c.dst[(long)dr * c.pitch + dc] = dc < c.cols ? c.src[(long)dr * c.ld + dc] : 0.0f;
+    }
+  }
+}
+
 typedef void (*F32Fn)(F32Batch);
 struct F32Cfg {
   int bm, bn;
@@ -588,8 +628,33 @@ int occ(int c, int v) {
 bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
 
 int g_f32_at = 1;  // large single FP32 products through the transposed-A kernel
+int g_f32_plan_cfg = -1;  // FP32 plans' configuration without autotuning (-1: by work)
 SharedWorkspace g_f32_at_ws;
 int g_f32_at_occ = 0;
+
+constexpr size_t kF32AtLds = 2 * 2 * 16 * 512;  // two stages of A and B slabs
+
+int f32_at_run(const F32Batch &b, int grid, hipStream_t stream) {
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_f32_at_occ) {
+      (void)hipFuncSetAttribute(reinterpret_cast<const void *>(f32_at_128x128),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kF32AtLds);
+      int n = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void *>(f32_at_128x128), 256,
+                                                       kF32AtLds) != hipSuccess ||
+          n < 1)
+        n = 1;
+      g_f32_at_occ = n;
+    }
+  }
+  int G = grid > 0 ? grid : g_f32_at_occ * phpc_device().cus;
+  if (G > b.total_tiles) G = b.total_tiles;
+  if (G < 1) G = 1;
+  hipLaunchKernelGGL(f32_at_128x128, dim3(G), dim3(256), kF32AtLds, stream, b);
+  PHPC_HIP(hipGetLastError());
+  return PHPC_OK;
+}
 
 // One large FP32 GEMM: A staged transposed (and B to a pitch of 4 floats when its rows are
 // not 16-byte aligned), then the transposed-A kernel over 128 x 128 tiles. With an aligned B
@@ -624,48 +689,124 @@ int f32_launch_transposed(F32Desc d, hipStream_t stream) {
   d.tiles_n = (d.n + 127) / 128;
   d.tile_start = 0;
   const int total_tiles = ((d.m + 127) / 128) * d.tiles_n;
-  constexpr size_t lds = 2 * 2 * 16 * 512;
-  if (!g_f32_at_occ) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(f32_at_128x128),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void *>(f32_at_128x128), 256, lds) !=
-            hipSuccess ||
-        n < 1)
-      n = 1;
-    g_f32_at_occ = n;
-  }
   F32Batch b{};
   b.ndesc = 1;
   b.total_tiles = total_tiles;
   b.d0 = d;
-  int G = g_f32_at_occ * phpc_device().cus;
-  if (G > total_tiles) G = total_tiles;
-  hipLaunchKernelGGL(f32_at_128x128, dim3(G), dim3(256), lds, stream, b);
-  PHPC_HIP(hipGetLastError());
+  if (int rc = f32_at_run(b, 0, stream)) return rc;
   g_f32_at_ws.mark_used(stream);
   return PHPC_OK;
 }
 
 }  // namespace
 
-int phpc_f32_num_configs() { return kNumF32; }
+static_assert(kF32AtConfig == kNumF32, "the transposed-A configuration follows the register-staged ones");
+
+int phpc_f32_num_configs() { return kNumF32 + 1; }
 
 void phpc_f32_config_tile(int id, int *bm, int *bn) {
-  *bm = kF32[id].bm;
-  *bn = kF32[id].bn;
+  *bm = id == kF32AtConfig ? 128 : kF32[id].bm;
+  *bn = id == kF32AtConfig ? 128 : kF32[id].bn;
 }
 
 // Fills tiles_n / tile_start of descs for config `id`; returns the total tile count.
 int phpc_f32_plan_tiles(F32Desc *descs, int ndesc, int id) {
+  int bm, bn;
+  phpc_f32_config_tile(id, &bm, &bn);
   int start = 0;
   for (int i = 0; i < ndesc; ++i) {
     F32Desc &d = descs[i];
-    d.tiles_n = (d.n + kF32[id].bn - 1) / kF32[id].bn;
+    d.tiles_n = (d.n + bn - 1) / bn;
     d.tile_start = start;
-    start += ((d.m + kF32[id].bm - 1) / kF32[id].bm) * d.tiles_n;
+    start += ((d.m + bm - 1) / bm) * d.tiles_n;
   }
   return start;
+}
+
+bool phpc_f32_at_ok(const F32Desc *descs, int ndesc) {
+  for (int i = 0; i < ndesc; ++i) {
+    const F32Desc &d = descs[i];
+    const long ldat = ((long)d.m + 3) / 4 * 4;
+    const long ldb = (!aligned16(d.B) || d.ldb % 4) ? ((long)d.n + 3) / 4 * 4 : d.ldb;
+    if (16L * ldat * 4 >= 0x7fffffffL || 16L * ldb * 4 >= 0x7fffffffL) return false;
+  }
+  return true;
+}
+
+void F32Staging::plan(const F32Desc *descs, int ndesc) {
+  copies.clear();
+  off.clear();
+  a_copy.assign(ndesc, -1);
+  b_copy.assign(ndesc, -1);
+  total = 0;
+  blocks = 0;
+  auto find_or_add = [&](const float *src, long ld, int rows, int cols, bool tr) {
+    for (size_t j = 0; j < copies.size(); ++j) {
+      const F32StageCopy &c = copies[j];
+      if (c.src == src && c.ld == ld && c.rows == rows && c.cols == cols && (c.transpose != 0) == tr) return (int)j;
+    }
+    F32StageCopy c{};
+    c.src = src;
+    c.ld = ld;
+    c.rows = rows;
+    c.cols = cols;
+    c.pitch = ((long)(tr ? rows : cols) + 3) / 4 * 4;
+    c.transpose = tr ? 1 : 0;
+    c.tiles_x = (int)((c.pitch + 31) / 32);
+    c.block_start = blocks;
+    blocks += c.tiles_x * (((tr ? cols : rows) + 31) / 32);
+    copies.push_back(c);
+    off.push_back(total);
+    const size_t n = (size_t)(tr ? cols : rows) * c.pitch;
+    total += (n + 63) / 64 * 64;  // 256-byte aligned copies
+    return (int)copies.size() - 1;
+  };
+  for (int i = 0; i < ndesc; ++i) {
+    const F32Desc &d = descs[i];
+    if (d.m <= 0 || d.n <= 0 || d.k <= 0) continue;
+    a_copy[i] = find_or_add(d.A, d.lda, d.m, d.k, true);
+    if (!aligned16(d.B) || d.ldb % 4) b_copy[i] = find_or_add(d.B, d.ldb, d.k, d.n, false);
+  }
+}
+
+void F32Staging::apply(float *ws, F32Desc *descs, int ndesc) {
+  for (size_t j = 0; j < copies.size(); ++j) copies[j].dst = ws + off[j];
+  for (int i = 0; i < ndesc; ++i) {
+    if (a_copy[i] >= 0) {
+      descs[i].A = copies[a_copy[i]].dst;
+      descs[i].lda = copies[a_copy[i]].pitch;
+    }
+    if (b_copy[i] >= 0) {
+      descs[i].B = copies[b_copy[i]].dst;
+      descs[i].ldb = copies[b_copy[i]].pitch;
+    }
+  }
+}
+
+int F32Staging::run(const F32StageCopy *copies_dev, hipStream_t stream) const {
+  if (copies.empty() || blocks <= 0) return PHPC_OK;
+  hipLaunchKernelGGL(f32_stage_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, copies_dev,
+                     (int)copies.size());
+  PHPC_HIP(hipGetLastError());
+  return PHPC_OK;
+}
+
+int phpc_f32_launch_at(const F32Desc *descs_dev, const F32Desc *descs_host, int ndesc, int total_tiles, int grid,
+                       hipStream_t stream) {
+  if (ndesc <= 0 || total_tiles <= 0) return PHPC_OK;
+  for (int i = 0; i < ndesc; ++i) {
+    const F32Desc &d = descs_host[i];
+    if (d.lda % 4 || d.ldb % 4 || !aligned16(d.A) || !aligned16(d.B) || d.lda < ((long)d.m + 3) / 4 * 4)
+      return phpc_set_error(PHPC_ERR_ARG, "transposed-A FP32 launch needs staged operands");
+    if (16L * d.lda * 4 >= 0x7fffffffL || 16L * d.ldb * 4 >= 0x7fffffffL)
+      return phpc_set_error(PHPC_ERR_ARG, "leading dimension too large");
+  }
+  F32Batch b{};
+  b.descs = descs_dev;
+  b.ndesc = ndesc;
+  b.total_tiles = total_tiles;
+  b.d0 = descs_host[0];
+  return f32_at_run(b, grid, stream);
 }
 
 bool phpc_f32_vec4_ok(const F32Desc *descs, int ndesc) {
@@ -718,6 +859,7 @@ int phpc_f32x3_launch(const F32Desc *descs_dev, const F32Desc *descs_host, int n
 int phpc_f32_launch(const F32Desc *descs_dev, const F32Desc *descs_host, int ndesc, int total_tiles, int id, bool vec4,
                     int grid, hipStream_t stream) {
   if (ndesc <= 0 || total_tiles <= 0) return PHPC_OK;
+  if (id < 0 || id >= kNumF32) return phpc_set_error(PHPC_ERR_ARG, "FP32 config %d is not register-staged", id);
   for (int i = 0; i < ndesc; ++i) {
     const F32Desc &d = descs_host[i];
     if ((long)kF32[id].bm * d.lda * 4 >= 0x7fffffffL || 16L * d.ldb * 4 >= 0x7fffffffL)
@@ -782,3 +924,17 @@ extern "C" int phpc_sgemm_device(const float *dA, long lda, const float *dB, lon
 }
 
 void phpc_f32_set_transposed_a(int on) { g_f32_at = on ? 1 : 0; }
+
+int phpc_f32_set_plan_config(int id) {
+  if (id < -1 || id > kF32AtConfig) return phpc_set_error(PHPC_ERR_ARG, "FP32 config %d out of range", id);
+  g_f32_plan_cfg = id;
+  return phpc_set_error(PHPC_OK, nullptr);
+}
+
+int phpc_f32_pick_plan_config(const F32Desc *descs, int ndesc) {
+  if (g_f32_plan_cfg >= 0 && (g_f32_plan_cfg != kF32AtConfig || phpc_f32_at_ok(descs, ndesc))) return g_f32_plan_cfg;
+  long work = 0;
+  for (int i = 0; i < ndesc; ++i) work += (long)((descs[i].m + 127) / 128) * ((descs[i].n + 127) / 128);
+  if (work < 2L * phpc_device().cus) return 0;
+  return g_f32_at && phpc_f32_at_ok(descs, ndesc) ? kF32AtConfig : 2;
+}

@@ -1305,6 +1305,8 @@ extern "C" int phpc_set_tuning(int key, int value) {
     case 6:
       phpc_f32_set_transposed_a(value);
       break;
+    case 7:
+      return phpc_f32_set_plan_config(value);
     default:
       return phpc_set_error(PHPC_ERR_ARG, "unknown tuning key %d", key);
   }

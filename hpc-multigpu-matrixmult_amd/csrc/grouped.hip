@@ -2,6 +2,7 @@
 #include "phpc_grouped.h"
 #include "phpc_internal.h"
 
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <map>
@@ -25,6 +26,12 @@ struct phpc_plan {
   F64Staging staging;
   std::vector<GemmDesc> d64_at;
   double *at_ws = nullptr;
+  // the FP32 transposed-A configuration: staged copies, their list on the device, and the
+  // descriptors that point at them
+  F32Staging staging32;
+  std::vector<F32Desc> d32_at;
+  float *at32_ws = nullptr;
+  F32StageCopy *copies_dev = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t graph_stream = nullptr;
@@ -66,6 +73,36 @@ int plan_prepare_at(phpc_plan *p) {
   return PHPC_OK;
 }
 
+int plan_prepare_at32(phpc_plan *p) {
+  if (p->at32_ws) return PHPC_OK;
+  p->staging32.plan(p->d32.data(), p->count);
+  const size_t ws_bytes = p->staging32.total * sizeof(float);
+  const size_t list_bytes = p->staging32.copies.size() * sizeof(F32StageCopy);
+  if (hipMalloc(&p->at32_ws, ws_bytes) != hipSuccess) {
+    p->at32_ws = nullptr;
+    return phpc_set_error(PHPC_ERR_MEM, "plan FP32 staging workspace allocation failed (%zu bytes)", ws_bytes);
+  }
+  if (list_bytes && hipMalloc(&p->copies_dev, list_bytes) != hipSuccess) {
+    p->copies_dev = nullptr;
+    (void)hipFree(p->at32_ws);
+    p->at32_ws = nullptr;
+    return phpc_set_error(PHPC_ERR_MEM, "plan FP32 staging list allocation failed");
+  }
+  p->d32_at = p->d32;
+  p->staging32.apply(p->at32_ws, p->d32_at.data(), p->count);
+  if (list_bytes)
+    PHPC_HIP(hipMemcpy(p->copies_dev, p->staging32.copies.data(), list_bytes, hipMemcpyHostToDevice));
+  return PHPC_OK;
+}
+
+void plan_release_at32(phpc_plan *p) {
+  if (p->copies_dev) (void)hipFree(p->copies_dev);
+  if (p->at32_ws) (void)hipFree(p->at32_ws);
+  p->copies_dev = nullptr;
+  p->at32_ws = nullptr;
+  p->d32_at.clear();
+}
+
 // (re)plan tiles for config id and upload the descriptors
 int plan_upload(phpc_plan *p, int id, hipStream_t s) {
   size_t bytes;
@@ -80,6 +117,13 @@ int plan_upload(phpc_plan *p, int id, hipStream_t s) {
     p->total_tiles = phpc_f64_plan_tiles(p->d64.data(), p->count, id);
     bytes = sizeof(GemmDesc) * p->count;
     src = p->d64.data();
+  } else if (p->dtype == PHPC_F32 && id == kF32AtConfig) {
+    if (int rc = plan_prepare_at32(p)) return rc;
+    p->total_tiles = phpc_f32_plan_tiles(p->d32_at.data(), p->count, id);
+    phpc_f32_plan_tiles(p->d32.data(), p->count, id);
+    p->vec = true;
+    bytes = sizeof(F32Desc) * p->count;
+    src = p->d32_at.data();
   } else {
     // the 3 x BF16 kernel has one tile shape, 128 x 128 (config 2's)
     p->total_tiles = phpc_f32_plan_tiles(p->d32.data(), p->count, p->dtype == PHPC_F32X3 ? 2 : id);
@@ -102,6 +146,10 @@ int launch(phpc_plan *p, hipStream_t s) {
   }
   if (p->dtype == PHPC_F64)
     return phpc_f64_grouped_dev((const GemmDesc *)p->descs_dev, p->d64.data(), p->count, p->grid, p->config, s);
+  if (p->dtype == PHPC_F32 && p->config == kF32AtConfig) {
+    if (int rc = p->staging32.run(p->copies_dev, s)) return rc;
+    return phpc_f32_launch_at((const F32Desc *)p->descs_dev, p->d32_at.data(), p->count, p->total_tiles, p->grid, s);
+  }
   if (p->dtype == PHPC_F32X3)
     return phpc_f32x3_launch((const F32Desc *)p->descs_dev, p->d32.data(), p->count, p->total_tiles, p->vec, p->grid,
                              s);
@@ -133,6 +181,12 @@ extern "C" int phpc_plan_create(int dtype, const phpc_gemm_item *items, int coun
     if (it.m == 0 || it.n == 0 || it.k == 0) continue;
     live.push_back(it);
   }
+  // Items in decreasing k: a tile's time grows with k, and the persistent workgroups take
+  // tiles round-robin from the flat list, so tiles of similar cost then share each round
+  // and the workgroups finish together (on log-uniform k, the slowest of 512 workgroups
+  // would otherwise carry about 18% more than the mean). Outputs do not depend on the order.
+  std::stable_sort(live.begin(), live.end(),
+                   [](const phpc_gemm_item &a, const phpc_gemm_item &b) { return a.k > b.k; });
   p->count = (int)live.size();
   bool mp = false;
   for (const auto &it : live) {
@@ -177,11 +231,8 @@ extern "C" int phpc_plan_create(int dtype, const phpc_gemm_item *items, int coun
   // transposed-A kernel when B allows 16-byte loads (the plan then keeps a copy of A)
   int cfg = dtype == PHPC_F64 ? phpc_f64_pick_config_at(p->d64.data(), p->count) : 0;
   const bool at_ok = dtype == PHPC_F64 && phpc_f64_trans_a_ok(p->d64.data(), p->count);
-  if (dtype == PHPC_F32) {
-    long work = 0;
-    for (const auto &d : p->d32) work += (long)((d.m + 127) / 128) * ((d.n + 127) / 128);
-    cfg = work >= 2L * phpc_device().cus ? 2 : 0;
-  }
+  const bool at32_ok = dtype == PHPC_F32 && phpc_f32_at_ok(p->d32.data(), p->count);
+  if (dtype == PHPC_F32) cfg = phpc_f32_pick_plan_config(p->d32.data(), p->count);
   if (dtype == PHPC_F32X3) cfg = 2;
   if (flags & PHPC_PLAN_AUTOTUNE) {
     const std::string sig = signature(dtype, live.data(), (int)live.size());
@@ -218,6 +269,7 @@ extern "C" int phpc_plan_create(int dtype, const phpc_gemm_item *items, int coun
         if (dtype == PHPC_F64 && mp && !phpc_f64_config_panels(c)) continue;
         if (dtype == PHPC_F64 && phpc_f64_config_trans_a(c) && !at_ok) continue;
         if (dtype == PHPC_F32X3 && c != 2) continue;
+        if (dtype == PHPC_F32 && c == kF32AtConfig && !at32_ok) continue;
         if (plan_upload(p, c, s) != PHPC_OK) continue;
         p->grid = 0;
         if (launch(p, s) != PHPC_OK) continue;
@@ -257,6 +309,10 @@ extern "C" int phpc_plan_create(int dtype, const phpc_gemm_item *items, int coun
     (void)hipFree(p->at_ws);
     p->at_ws = nullptr;
     p->d64_at.clear();
+  }
+  if (p->at32_ws && cfg != kF32AtConfig) {
+    (void)hipStreamSynchronize(s);
+    plan_release_at32(p);
   }
   p->grid = 0;
   *out = p;
@@ -313,5 +369,6 @@ extern "C" void phpc_plan_destroy(phpc_plan_t p) {
   }
   if (p->descs_dev) (void)hipFree(p->descs_dev);
   if (p->at_ws) (void)hipFree(p->at_ws);
+  plan_release_at32(p);
   delete p;
 }

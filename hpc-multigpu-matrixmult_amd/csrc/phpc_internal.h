@@ -69,6 +69,45 @@ int phpc_f32_num_configs();
 void phpc_f32_config_tile(int id, int *bm, int *bn);
 int phpc_f32_plan_tiles(F32Desc *descs, int ndesc, int id);
 void phpc_f32_set_transposed_a(int on);
+// configuration of FP32 plans created without autotuning: -1 = by work (at least two
+// 128x128 tiles per CU: the transposed-A configuration 3, or 2 where its staged copies would
+// not fit its 32-bit offsets or tuning key 6 is off; else 0, 64x64); 0-3 = that
+// configuration (3 where it applies)
+int phpc_f32_set_plan_config(int id);
+int phpc_f32_pick_plan_config(const struct F32Desc *descs, int ndesc);
+// FP32 configuration ids: 0-2 register-staged (64x64, 128x64, 128x128); kF32AtConfig is the
+// transposed-A direct-to-LDS kernel (128x128), which reads staged operand copies.
+constexpr int kF32AtConfig = 3;
+// One operand copy of a grouped FP32 transposed-A launch, filled by one batched kernel:
+// transpose: dst[c][r] = src[r][c] for r < rows, c < cols, zero for rows <= r < pitch;
+// otherwise dst[r][c] = src[r][c], zero for cols <= c < pitch. 32 x 32 tiles of dst,
+// tiles_x across a dst row; block_start is the first tile's index in the batched grid.
+struct F32StageCopy {
+  const float *src;
+  long ld;
+  int rows, cols;
+  float *dst;
+  long pitch;
+  int transpose;
+  int tiles_x;
+  int block_start;
+};
+struct F32Staging {
+  std::vector<F32StageCopy> copies;  // dst set by apply
+  std::vector<size_t> off;           // floats into the workspace, per copy
+  std::vector<int> a_copy, b_copy;   // per descriptor; b_copy -1: B is read in place
+  size_t total = 0;                  // floats
+  int blocks = 0;                    // tiles of the batched copy kernel
+  void plan(const F32Desc *descs, int ndesc);
+  void apply(float *ws, F32Desc *descs, int ndesc);
+  int run(const F32StageCopy *copies_dev, hipStream_t stream) const;
+};
+// the descriptors fit the transposed-A kernel's 32-bit buffer offsets once staged
+bool phpc_f32_at_ok(const F32Desc *descs, int ndesc);
+// transposed-A kernel over descriptors already pointing at staged copies (tiles planned
+// with config kF32AtConfig); grid 0 = persistent default
+int phpc_f32_launch_at(const F32Desc *descs_dev, const F32Desc *descs_host, int ndesc, int total_tiles, int grid,
+                       hipStream_t stream);
 bool phpc_f32_vec4_ok(const F32Desc *descs, int ndesc);
 int phpc_f32x3_launch(const F32Desc *descs_dev, const F32Desc *descs_host, int ndesc, int total_tiles, bool vec4,
                       int grid, hipStream_t stream);

@@ -328,11 +328,13 @@ def num_configs():
 
 
 TUNE_F32_TRANSPOSED_A = 6
+TUNE_F32_PLAN_CONFIG = 7
 
 
 def set_tuning(key, value):
     """One tuning knob (phpc_set_tuning): keys 0-5 pick FP64 configs, key 6 switches large
-    single FP32 products between the transposed-A kernel (1) and the register-staged one (0)."""
+    single FP32 products between the transposed-A kernel (1) and the register-staged one (0),
+    key 7 sets the FP32 configuration of plans created without autotuning (-1 = by work)."""
     check(lib().phpc_set_tuning(key, value))
 
 

@@ -92,7 +92,11 @@ int phpc_num_configs(void);
  * config for large single-panel / panel-grouped problems through the entry points that
  * may stage a transposed copy of A (configs 10-12 read such a copy), key 6 = large
  * single FP32 products through phpc_sgemm_device on the transposed-A FP32 kernel (1,
- * default) or on the register-staged one (0). */
+ * default) or on the register-staged one (0); 0 also keeps FP32 plans created without
+ * autotuning off configuration 3; key 7 = FP32 configuration of plans created
+ * without autotuning (-1 = by work, default: 3 for at least two 128x128 tiles per CU, else
+ * 0; 0-2 = register-staged 64x64, 128x64, 128x128; 3 = transposed-A 128x128, with the plan
+ * keeping staged operand copies). */
 int phpc_set_tuning(int key, int value);
 /* Describe configuration `id`: macro tile bm x bn, threads per workgroup. */
 int phpc_config_info(int id, int *bm, int *bn, int *threads);

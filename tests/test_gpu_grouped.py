@@ -69,6 +69,71 @@ def test_f32_vec4_loads(shapes):
         assert np.all(np.abs(dC.cpu().numpy().astype(np.float64) - ref) <= f32_bound(A, B, C0))
 
 
+def _f32_plan_with_config(cfg, gemms):
+    phpc.set_tuning(phpc.TUNE_F32_PLAN_CONFIG, cfg)
+    try:
+        return phpc.Plan(phpc.F32, gemms)
+    finally:
+        phpc.set_tuning(phpc.TUNE_F32_PLAN_CONFIG, -1)
+
+
+def test_f32_plan_transposed_a_bit_identical_and_replays():
+    """FP32 plans on the transposed-A configuration: one batched kernel stages every A
+    transposed (an A shared by two items once) and every B whose rows are not 16-byte
+    aligned, then one persistent launch runs all items. The kernel takes k in the same pairs
+    and order as the register-staged configurations, so the results equal configuration 2's
+    bit for bit, within the FP32 bound; captured replays (eager work in between) equal eager
+    launches."""
+    import torch
+    shapes = [(300, 257, 200), (128, 128, 16), (1, 1, 1), (77, 4096, 3), (1000, 17, 999), (512, 768, 256),
+              (2049, 130, 64), (64, 64, 2)]
+    host = []
+    for i, (m, n, k) in enumerate(shapes):
+        host.append((O.uniform((m, k), 70 + i).astype(np.float32), O.uniform((k, n), 80 + i).astype(np.float32),
+                     O.uniform((m, n), 90 + i).astype(np.float32)))
+    # one more item sharing the first item's A, with a column-offset (unaligned) view of B
+    Bw = O.uniform((200, 70), 99).astype(np.float32)
+    host.append((host[0][0], Bw[:, 1:66], O.uniform((300, 65), 98).astype(np.float32)))
+
+    def make():
+        dA0 = torch.from_numpy(host[0][0]).cuda()
+        g = []
+        for i, (A, B, C0) in enumerate(host):
+            dA = dA0 if i in (0, len(host) - 1) else torch.from_numpy(A).cuda()
+            dB = torch.from_numpy(np.ascontiguousarray(B)).cuda() if i < len(host) - 1 else \
+                torch.from_numpy(Bw).cuda()[:, 1:66]
+            g.append((dA, dB, torch.from_numpy(C0.copy()).cuda()))
+        return g
+
+    ga, gr, ge = make(), make(), make()
+    pa = _f32_plan_with_config(3, ga)
+    pr = _f32_plan_with_config(2, gr)
+    pe = _f32_plan_with_config(3, ge)
+    assert pa.info()["config"] == 3 and pr.info()["config"] == 2
+    s = torch.cuda.Stream()
+    junk = torch.randn(512, 512, device="cuda")
+    for rep in range(3):
+        with torch.cuda.stream(s):
+            pa.graph_launch(s)
+        s.synchronize()
+        junk = junk @ junk.T / 512.0
+        torch.cuda.synchronize()
+        pr.launch()
+        pe.launch()
+        torch.cuda.synchronize()
+        for (_, _, c_a), (_, _, c_r), (_, _, c_e) in zip(ga, gr, ge):
+            assert torch.equal(c_a.view(torch.int32), c_r.view(torch.int32)), rep
+            assert torch.equal(c_a.view(torch.int32), c_e.view(torch.int32)), rep
+    for (_, _, dC), (A, B, C0) in zip(ga, host):
+        ref = C0.astype(np.float64) + 3 * (A.astype(np.float64) @ B.astype(np.float64))
+        # three launches, each within (k + 2) u (|C before| + |A||B|)
+        bound = 3 * (A.shape[1] + 2) * 2.0 ** -24 * (np.abs(C0) + 3 * np.abs(A.astype(np.float64)) @
+                                                    np.abs(B.astype(np.float64)))
+        assert np.all(np.abs(dC.cpu().numpy().astype(np.float64) - ref) <= bound)
+    for p in (pa, pr, pe):
+        p.close()
+
+
 def test_f64_grouped_items_bitexact():
     import torch
     shapes = [(130, 70, 90), (1, 1, 1), (257, 3, 33), (64, 64, 64), (300, 129, 17)]
