@@ -128,7 +128,8 @@ def grouped_bench(phpc, seed=21):
     out["fp64_n10000_grid3x3"] = {"tflops": round(2.0 * N ** 3 / (ms * 1e-3) / 1e12, 3), "ms": round(ms, 3),
                                   "blocks": "3334/3333/3333", "panel": N // 3, "config": info["config"],
                                   "tiles": info["tiles"], "err_cw_max_sampled": cw, "err_nw_sampled": nw,
-                                  "launch": "one persistent kernel per pass, hipGraph replay"}
+                                  "launch": "one persistent panel-grouped launch for all 9 rank blocks and all "
+                                            "k panels (3333, 3333, 3333, 1), hipGraph replay"}
     plan.close()
     del A, B, C, items
     torch.cuda.empty_cache()
