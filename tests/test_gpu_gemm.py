@@ -204,3 +204,59 @@ def test_transposed_a_configs_on_ragged_shapes(cfg, m, k, n):
     finally:
         phpc.set_launch_policy(0, -1, 0)
     assert np.array_equal(bits(dC.cpu().numpy()), bits(want))
+
+
+def special_inputs(m, k, n, seed):
+    """Uniform operands seeded with IEEE special cases: rows and columns of tiny values whose
+    products and partial sums are subnormal, subnormal entries, signed zeros, infinities and
+    NaNs in A, B and C0."""
+    A, B, C0 = O.uniform((m, k), seed), O.uniform((k, n), seed + 1), O.uniform((m, n), seed + 2)
+    A[0, :] *= 1e-160
+    B[:, 0] *= 1e-160
+    C0[0, 0] = 0.0               # C[0, 0]: sums of 1e-320 products, subnormal throughout
+    A[1, :] = 5e-324 * np.arange(1, k + 1)  # subnormal entries; FMAs round in the subnormal range
+    C0[1, :] = 0.0
+    A[2, :] = -0.0
+    C0[2, :] = -0.0              # -0 + (+0 from a chain of -0 products) = +0
+    B[:, 3] = -0.0
+    A[4, 5] = np.inf
+    A[4, 6] = -np.inf
+    B[5, 8] = B[6, 8] = 1.0      # C[4, 8]: inf - inf = NaN; the rest of row 4 is +-inf or NaN
+    B[7, 9] = np.nan
+    C0[6, 10] = np.inf
+    C0[7, 11] = np.nan
+    C0[8, 12] = 4.9e-324
+    return A, B, C0
+
+
+def same_bits_or_both_nan(got, want):
+    gn, wn = np.isnan(got), np.isnan(want)
+    return np.array_equal(gn, wn) and np.array_equal(bits(np.where(gn, 0.0, got)), bits(np.where(wn, 0.0, want)))
+
+
+@pytest.mark.parametrize("cfg", [-1] + list(range(13)))
+def test_special_values_match_reference_kernel(cfg):
+    """IEEE special cases through every tile configuration: subnormal products and sums are
+    kept (no flush to zero), signed zeros, infinities and NaNs propagate as in the reference
+    kernel's FMA chain. NaN payloads are not compared."""
+    import torch
+    m, k, n = 70, 45, 66
+    A, B, C0 = special_inputs(m, k, n, 61)
+    want = O.kernel_emul(A, B, C0.copy())
+    assert np.isnan(want[4, 8]) and np.isnan(want[7, 11]) and 0 < abs(want[0, 0]) < 2.3e-308
+    assert np.count_nonzero((np.abs(want[1]) < 2.3e-308) & (want[1] != 0)) > n // 2
+    assert not np.signbit(want[2, 0])
+    if cfg >= phpc.num_configs():
+        pytest.skip("no such configuration")
+    dA, dB = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
+    dC = torch.from_numpy(C0.copy()).cuda()
+    phpc.set_launch_policy(0, cfg, 0)
+    try:
+        phpc.dgemm_device(dA, dB, dC)
+        torch.cuda.synchronize()
+    finally:
+        phpc.set_launch_policy(0, -1, 0)
+    assert same_bits_or_both_nan(dC.cpu().numpy(), want), cfg
+    C = C0.copy()
+    phpc.gemm_cuda(A, B, C, grid_width=4, grid_height=4, block_width=16)
+    assert same_bits_or_both_nan(C, want)
