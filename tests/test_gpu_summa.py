@@ -268,3 +268,23 @@ def test_chained_transposed_a_matches_register_staged_full_matrix():
             phpc.set_launch_policy(0, -1, 0)
         assert torch.equal(dC1.view(torch.int64), dC2.view(torch.int64)), dist
         del dC1, dC2
+
+
+@pytest.mark.parametrize("mode", ["fused", "stepwise", "chained"])
+def test_summa_special_values_match_oracle(mode):
+    """IEEE special cases through the SUMMA schedules on an uneven 3 x 3 grid: subnormal
+    sums, signed zeros, infinities and NaNs come out as in the oracle's restatement (NaN
+    positions compared, payloads not)."""
+    import torch
+    from test_gpu_gemm import special_inputs, same_bits_or_both_nan
+    N, p, q, panel = 100, 3, 3, 11
+    A, B, C0 = special_inputs(N, N, N, 71)
+    if mode == "chained":
+        want = O.kernel_emul(A, B, C0.copy())
+    else:
+        want = O.summa_emul(A, B, C0.copy(), p, q, panel)
+    dC = torch.from_numpy(C0.copy()).cuda()
+    m = {"fused": phpc.SUMMA_FUSED, "stepwise": phpc.SUMMA_STEPWISE, "chained": phpc.SUMMA_CHAINED}[mode]
+    phpc.summa_device(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(), dC, p, q, panel=panel, mode=m)
+    torch.cuda.synchronize()
+    assert same_bits_or_both_nan(dC.cpu().numpy(), want), mode
