@@ -183,8 +183,8 @@ def grouped_bench(phpc, seed=21):
                              "flops": flops, "config": info["config"], "tiles": info["tiles"],
                              "tuned_ms": round(info["tuned_ms"], 3), "err_cw_max_sampled": worst,
                              "fp32_mfma_util": round(flops / (ms * 1e-3) / 157.3e12, 4),
-                             "launch": ("one batched staging kernel (every A transposed, unaligned B rows "
-                                        "padded) + one persistent transposed-A kernel for all 512 GEMMs"
+                             "launch": ("one batched kernel transposing every A + one persistent "
+                                        "transposed-A kernel for all 512 GEMMs (B read in place)"
                                         if info["config"] == 3 else
                                         "one persistent kernel for all 512 GEMMs") +
                                        ", items in decreasing k, hipGraph replay"}

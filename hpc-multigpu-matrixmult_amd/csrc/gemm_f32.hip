@@ -403,12 +403,18 @@ __device__ __forceinline__ void f32_wait_vm() {
   __builtin_amdgcn_s_waitcnt(0x0F70 | (N & 0xF) | (((N >> 4) & 3) << 14));
 }
 
+// B4: B rows need no alignment. Each B k row then arrives as two 4-byte-per-lane loads of 64
+// columns (lane i of half h at position 64 h + i holds column (64 h + i) ^ 32 on odd rows,
+// the same permutation as the 16-byte chunks' XOR by 8), exact to the last column of the
+// row; A still arrives in 1 KiB blocks.
+template <bool B4>
 __device__ __forceinline__ void f32_at_body(const F32Batch &batch) {
   constexpr int BM = 128, BN = 128, WTM = 64, WTN = 64, TM = 2, TN = 2;
   constexpr int ROW_BYTES = 512;                  // one 128-float k row
   constexpr int A_BYTES = BK * ROW_BYTES;         // 16 rows
   constexpr int STAGE_BYTES = 2 * A_BYTES;        // A then B
   constexpr int A_BLOCKS = A_BYTES / 1024, BLOCKS = 2 * A_BLOCKS, PER = BLOCKS / 4;
+  constexpr int B4_PER = 2 * BK / 4;              // 4-byte B loads per wave and slab
   extern __shared__ __attribute__((aligned(1024))) char smem[];
   const int tid = threadIdx.x;
   const int wave = uni(tid >> 6);
@@ -445,6 +451,16 @@ __device__ __forceinline__ void f32_at_body(const F32Batch &batch) {
       const int rem = isA ? m_rem : n_rem;
       goff[u] = c < rem ? (uint32_t)(((long)r * (isA ? ldat : ldb) + c) * 4) : kOOB;
     }
+    uint32_t goff_b4[B4 ? B4_PER : 1];
+    if constexpr (B4) {
+#pragma unroll
+      for (int v = 0; v < B4_PER; ++v) {
+        const int j = wave + v * 4;  // row j / 2, half j % 2
+        const int r = j >> 1, h = j & 1;
+        const int c = (64 * h + lane) ^ ((r & 1) << 5);
+        goff_b4[v] = c < n_rem ? (uint32_t)(((long)r * ldb + c) * 4) : kOOB;
+      }
+    }
     const int full_a = (int)(BK * ldat * 4), full_b = (int)(BK * ldb * 4);
     auto issue = [&](int sidx) {
       const int kb = sidx * BK;
@@ -456,11 +472,25 @@ __device__ __forceinline__ void f32_at_body(const F32Batch &batch) {
                                                           0x00020000);
       const rsrc_t rB =
           __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(Bg + (long)kb * ldb), (short)0, nb, 0x00020000);
+      if constexpr (B4) {
 #pragma unroll
-      for (int u = 0; u < PER; ++u) {
-        const int blk = wave + u * 4;
-        __builtin_amdgcn_raw_ptr_buffer_load_lds(blk < A_BLOCKS ? rA : rB, (lds_void *)(stage + blk * 1024), 16,
-                                                  goff[u], 0, 0, 0);
+        for (int u = 0; u < A_BLOCKS / 4; ++u) {
+          const int blk = wave + u * 4;
+          __builtin_amdgcn_raw_ptr_buffer_load_lds(rA, (lds_void *)(stage + blk * 1024), 16, goff[u], 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < B4_PER; ++v) {
+          const int j = wave + v * 4;
+          __builtin_amdgcn_raw_ptr_buffer_load_lds(rB, (lds_void *)(stage + A_BYTES + (j >> 1) * ROW_BYTES + (j & 1) * 256),
+                                                    4, goff_b4[v], 0, 0, 0);
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+          const int blk = wave + u * 4;
+          __builtin_amdgcn_raw_ptr_buffer_load_lds(blk < A_BLOCKS ? rA : rB, (lds_void *)(stage + blk * 1024), 16,
+                                                    goff[u], 0, 0, 0);
+        }
       }
     };
 
@@ -523,7 +553,8 @@ __device__ __forceinline__ void f32_at_body(const F32Batch &batch) {
   }
 }
 
-__global__ __launch_bounds__(256) void f32_at_128x128(F32Batch batch) { f32_at_body(batch); }
+__global__ __launch_bounds__(256) void f32_at_128x128(F32Batch batch) { f32_at_body<false>(batch); }
+__global__ __launch_bounds__(256) void f32_at_128x128_b4(F32Batch batch) { f32_at_body<true>(batch); }
 
 // At[k][m] = A[m][k] for m < M, k < K; 0 for M <= m < ldat
 __global__ __launch_bounds__(256) void f32_transpose_kernel(const float *__restrict__ A, long lda, int M, int Kd,
@@ -544,15 +575,7 @@ __global__ __launch_bounds__(256) void f32_transpose_kernel(const float *__restr
   }
 }
 
-// Bc[k][j] = B[k][j] for j < N, 0 for N <= j < ldbc
-__global__ __launch_bounds__(256) void f32_copy_pad_kernel(const float *__restrict__ B, long ldb, int Kd, int N,
-                                                           float *__restrict__ Bc, long ldbc) {
-  const int k = blockIdx.y;
-  for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < ldbc; j += (long)gridDim.x * blockDim.x)
-    Bc[(long)k * ldbc + j] = j < N ? B[(long)k * ldb + j] : 0.0f;
-}
-
-// All operand copies of a grouped transposed-A launch in one kernel: one workgroup per
+// All transposed A copies of a grouped transposed-A launch in one kernel: one workgroup per
 // 32 x 32 tile of a copy's destination, the copy found by binary search over block_start.
 __global__ __launch_bounds__(256) void f32_stage_kernel(const F32StageCopy *__restrict__ copies, int ncopies) {
   __shared__ float tile[32][33];
@@ -567,28 +590,19 @@ __global__ __launch_bounds__(256) void f32_stage_kernel(const F32StageCopy *__re
   }
   const F32StageCopy c = copies[lo];
   const int lb = b - c.block_start;
-  const int d0 = (lb / c.tiles_x) * 32;  // first destination row of the tile
-  const int e0 = (lb % c.tiles_x) * 32;  // first destination column
+  const int d0 = (lb / c.tiles_x) * 32;  // first destination row (k) of the tile
+  const int e0 = (lb % c.tiles_x) * 32;  // first destination column (m)
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-  if (c.transpose) {
-    // destination rows are source columns (k), destination columns source rows (m)
 #pragma unroll
-    for (int r = ty; r < 32; r += 8) {
-      const int sr = e0 + r, sc = d0 + tx;
-      tile[r][tx] = (sr < c.rows && sc < c.cols) ? c.src[(long)sr * c.ld + sc] : 0.0f;
-    }
-    __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int sr = e0 + r, sc = d0 + tx;
+    tile[r][tx] = (sr < c.rows && sc < c.cols) ? c.src[(long)sr * c.ld + sc] : 0.0f;
+  }
+  __syncthreads();
 #pragma unroll
-    for (int r = ty; r < 32; r += 8) {
-      const int dr = d0 + r, dc = e0 + tx;
-      if (dr < c.cols && dc < c.pitch) c.dst[(long)dr * c.pitch + dc] = tile[tx][r];
-    }
-  } else {
-#pragma unroll
-    for (int r = ty; r < 32; r += 8) {
-      const int dr = d0 + r, dc = e0 + tx;
-      if (dr < c.rows && dc < c.pitch) c.dst[(long)dr * c.pitch + dc] = dc < c.cols ? c.src[(long)dr * c.ld + dc] : 0.0f;
-    }
+  for (int r = ty; r < 32; r += 8) {
+    const int dr = d0 + r, dc = e0 + tx;
+    if (dr < c.cols && dc < c.pitch) c.dst[(long)dr * c.pitch + dc] = tile[tx][r];
   }
 }
 
@@ -630,45 +644,46 @@ bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) ==
 int g_f32_at = 1;  // large single FP32 products through the transposed-A kernel
 int g_f32_plan_cfg = -1;  // FP32 plans' configuration without autotuning (-1: by work)
 SharedWorkspace g_f32_at_ws;
-int g_f32_at_occ = 0;
+int g_f32_at_occ[2] = {0, 0};
 
 constexpr size_t kF32AtLds = 2 * 2 * 16 * 512;  // two stages of A and B slabs
 
-int f32_at_run(const F32Batch &b, int grid, hipStream_t stream) {
+// b4: some B rows are not 16-byte aligned (or their pitch is not a multiple of 4 floats);
+// the kernel then loads B 4 bytes per lane
+int f32_at_run(const F32Batch &b, bool b4, int grid, hipStream_t stream) {
+  const void *fn = b4 ? reinterpret_cast<const void *>(f32_at_128x128_b4) : reinterpret_cast<const void *>(f32_at_128x128);
+  int &occ_v = g_f32_at_occ[b4 ? 1 : 0];
   {
     std::lock_guard<std::mutex> lk(g_mu);
-    if (!g_f32_at_occ) {
-      (void)hipFuncSetAttribute(reinterpret_cast<const void *>(f32_at_128x128),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kF32AtLds);
+    if (!occ_v) {
+      (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kF32AtLds);
       int n = 0;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void *>(f32_at_128x128), 256,
-                                                       kF32AtLds) != hipSuccess ||
-          n < 1)
-        n = 1;
-      g_f32_at_occ = n;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 256, kF32AtLds) != hipSuccess || n < 1) n = 1;
+      occ_v = n;
     }
   }
-  int G = grid > 0 ? grid : g_f32_at_occ * phpc_device().cus;
+  int G = grid > 0 ? grid : occ_v * phpc_device().cus;
   if (G > b.total_tiles) G = b.total_tiles;
   if (G < 1) G = 1;
-  hipLaunchKernelGGL(f32_at_128x128, dim3(G), dim3(256), kF32AtLds, stream, b);
+  if (b4)
+    hipLaunchKernelGGL(f32_at_128x128_b4, dim3(G), dim3(256), kF32AtLds, stream, b);
+  else
+    hipLaunchKernelGGL(f32_at_128x128, dim3(G), dim3(256), kF32AtLds, stream, b);
   PHPC_HIP(hipGetLastError());
   return PHPC_OK;
 }
 
-// One large FP32 GEMM: A staged transposed (and B to a pitch of 4 floats when its rows are
-// not 16-byte aligned), then the transposed-A kernel over 128 x 128 tiles. With an aligned B
-// and a pitch of 4 floats, the 16-byte chunk holding a row's last column starts on a 16-byte
-// boundary and so lies in the same page as that column; its extra columns only feed output
-// columns that are never stored.
+bool b16_ok(const F32Desc &d) { return aligned16(d.B) && d.ldb % 4 == 0; }
+
+// One large FP32 GEMM: A staged transposed, then the transposed-A kernel over 128 x 128
+// tiles, B read in place (16-byte loads when its rows are 16-byte aligned with a pitch of 4
+// floats, else 4-byte loads). With 16-byte loads, the chunk holding a row's last column
+// starts on a 16-byte boundary and so lies in the same page as that column; its extra
+// columns only feed output columns that are never stored.
 int f32_launch_transposed(F32Desc d, hipStream_t stream) {
   const long ldat = ((long)d.m + 3) / 4 * 4;
-  const bool copy_b = !aligned16(d.B) || d.ldb % 4;
-  const long ldbc = ((long)d.n + 3) / 4 * 4;
-  const size_t a_elems = ((size_t)d.k * ldat + 63) / 64 * 64;
-  const size_t total = a_elems + (copy_b ? (size_t)d.k * ldbc : 0);
-  if (16L * ldat * 4 >= 0x7fffffffL || 16L * (copy_b ? ldbc : d.ldb) * 4 >= 0x7fffffffL ||
-      128L * d.ldc * 4 >= 0x7fffffffL)
+  const size_t total = ((size_t)d.k * ldat + 63) / 64 * 64;
+  if (16L * ldat * 4 >= 0x7fffffffL || 16L * d.ldb * 4 >= 0x7fffffffL)
     return phpc_set_error(PHPC_ERR_ARG, "leading dimension too large");
   std::lock_guard<std::mutex> lk(g_f32_at_ws.mu);
   g_f32_at_ws.after_previous(stream);
@@ -677,13 +692,6 @@ int f32_launch_transposed(F32Desc d, hipStream_t stream) {
   hipLaunchKernelGGL(f32_transpose_kernel, dim3((unsigned)((d.k + 31) / 32), (unsigned)((ldat + 31) / 32)), dim3(256),
                      0, stream, d.A, d.lda, d.m, d.k, ws, ldat);
   PHPC_HIP(hipGetLastError());
-  if (copy_b) {
-    hipLaunchKernelGGL(f32_copy_pad_kernel, dim3((unsigned)std::min<long>((ldbc + 255) / 256, 64), (unsigned)d.k),
-                       dim3(256), 0, stream, d.B, d.ldb, d.k, d.n, ws + a_elems, ldbc);
-    PHPC_HIP(hipGetLastError());
-    d.B = ws + a_elems;
-    d.ldb = ldbc;
-  }
   d.A = ws;
   d.lda = ldat;
   d.tiles_n = (d.n + 127) / 128;
@@ -693,7 +701,7 @@ int f32_launch_transposed(F32Desc d, hipStream_t stream) {
   b.ndesc = 1;
   b.total_tiles = total_tiles;
   b.d0 = d;
-  if (int rc = f32_at_run(b, 0, stream)) return rc;
+  if (int rc = f32_at_run(b, !b16_ok(d), 0, stream)) return rc;
   g_f32_at_ws.mark_used(stream);
   return PHPC_OK;
 }
@@ -727,8 +735,7 @@ bool phpc_f32_at_ok(const F32Desc *descs, int ndesc) {
   for (int i = 0; i < ndesc; ++i) {
     const F32Desc &d = descs[i];
     const long ldat = ((long)d.m + 3) / 4 * 4;
-    const long ldb = (!aligned16(d.B) || d.ldb % 4) ? ((long)d.n + 3) / 4 * 4 : d.ldb;
-    if (16L * ldat * 4 >= 0x7fffffffL || 16L * ldb * 4 >= 0x7fffffffL) return false;
+    if (16L * ldat * 4 >= 0x7fffffffL || 16L * d.ldb * 4 >= 0x7fffffffL) return false;
   }
   return true;
 }
@@ -737,35 +744,32 @@ void F32Staging::plan(const F32Desc *descs, int ndesc) {
   copies.clear();
   off.clear();
   a_copy.assign(ndesc, -1);
-  b_copy.assign(ndesc, -1);
   total = 0;
   blocks = 0;
-  auto find_or_add = [&](const float *src, long ld, int rows, int cols, bool tr) {
+  auto find_or_add = [&](const float *src, long ld, int rows, int cols) {
     for (size_t j = 0; j < copies.size(); ++j) {
       const F32StageCopy &c = copies[j];
-      if (c.src == src && c.ld == ld && c.rows == rows && c.cols == cols && (c.transpose != 0) == tr) return (int)j;
+      if (c.src == src && c.ld == ld && c.rows == rows && c.cols == cols) return (int)j;
     }
     F32StageCopy c{};
     c.src = src;
     c.ld = ld;
     c.rows = rows;
     c.cols = cols;
-    c.pitch = ((long)(tr ? rows : cols) + 3) / 4 * 4;
-    c.transpose = tr ? 1 : 0;
+    c.pitch = ((long)rows + 3) / 4 * 4;
     c.tiles_x = (int)((c.pitch + 31) / 32);
     c.block_start = blocks;
-    blocks += c.tiles_x * (((tr ? cols : rows) + 31) / 32);
+    blocks += c.tiles_x * ((cols + 31) / 32);
     copies.push_back(c);
     off.push_back(total);
-    const size_t n = (size_t)(tr ? cols : rows) * c.pitch;
+    const size_t n = (size_t)cols * c.pitch;
     total += (n + 63) / 64 * 64;  // 256-byte aligned copies
     return (int)copies.size() - 1;
   };
   for (int i = 0; i < ndesc; ++i) {
     const F32Desc &d = descs[i];
     if (d.m <= 0 || d.n <= 0 || d.k <= 0) continue;
-    a_copy[i] = find_or_add(d.A, d.lda, d.m, d.k, true);
-    if (!aligned16(d.B) || d.ldb % 4) b_copy[i] = find_or_add(d.B, d.ldb, d.k, d.n, false);
+    a_copy[i] = find_or_add(d.A, d.lda, d.m, d.k);
   }
 }
 
@@ -775,10 +779,6 @@ void F32Staging::apply(float *ws, F32Desc *descs, int ndesc) {
     if (a_copy[i] >= 0) {
       descs[i].A = copies[a_copy[i]].dst;
       descs[i].lda = copies[a_copy[i]].pitch;
-    }
-    if (b_copy[i] >= 0) {
-      descs[i].B = copies[b_copy[i]].dst;
-      descs[i].ldb = copies[b_copy[i]].pitch;
     }
   }
 }
@@ -794,19 +794,21 @@ int F32Staging::run(const F32StageCopy *copies_dev, hipStream_t stream) const {
 int phpc_f32_launch_at(const F32Desc *descs_dev, const F32Desc *descs_host, int ndesc, int total_tiles, int grid,
                        hipStream_t stream) {
   if (ndesc <= 0 || total_tiles <= 0) return PHPC_OK;
+  bool b4 = false;
   for (int i = 0; i < ndesc; ++i) {
     const F32Desc &d = descs_host[i];
-    if (d.lda % 4 || d.ldb % 4 || !aligned16(d.A) || !aligned16(d.B) || d.lda < ((long)d.m + 3) / 4 * 4)
-      return phpc_set_error(PHPC_ERR_ARG, "transposed-A FP32 launch needs staged operands");
+    if (d.lda % 4 || !aligned16(d.A) || d.lda < ((long)d.m + 3) / 4 * 4)
+      return phpc_set_error(PHPC_ERR_ARG, "transposed-A FP32 launch needs a staged A");
     if (16L * d.lda * 4 >= 0x7fffffffL || 16L * d.ldb * 4 >= 0x7fffffffL)
       return phpc_set_error(PHPC_ERR_ARG, "leading dimension too large");
+    if (!b16_ok(d)) b4 = true;
   }
   F32Batch b{};
   b.descs = descs_dev;
   b.ndesc = ndesc;
   b.total_tiles = total_tiles;
   b.d0 = descs_host[0];
-  return f32_at_run(b, grid, stream);
+  return f32_at_run(b, b4, grid, stream);
 }
 
 bool phpc_f32_vec4_ok(const F32Desc *descs, int ndesc) {

@@ -78,24 +78,23 @@ int phpc_f32_pick_plan_config(const struct F32Desc *descs, int ndesc);
 // FP32 configuration ids: 0-2 register-staged (64x64, 128x64, 128x128); kF32AtConfig is the
 // transposed-A direct-to-LDS kernel (128x128), which reads staged operand copies.
 constexpr int kF32AtConfig = 3;
-// One operand copy of a grouped FP32 transposed-A launch, filled by one batched kernel:
-// transpose: dst[c][r] = src[r][c] for r < rows, c < cols, zero for rows <= r < pitch;
-// otherwise dst[r][c] = src[r][c], zero for cols <= c < pitch. 32 x 32 tiles of dst,
-// tiles_x across a dst row; block_start is the first tile's index in the batched grid.
+// One transposed A copy of a grouped FP32 transposed-A launch, filled by one batched kernel:
+// dst[c][r] = src[r][c] for r < rows, c < cols, zero for rows <= r < pitch. 32 x 32 tiles of
+// dst, tiles_x across a dst row; block_start is the first tile's index in the batched grid.
+// B is read in place (the kernel loads unaligned B rows 4 bytes per lane).
 struct F32StageCopy {
   const float *src;
   long ld;
   int rows, cols;
   float *dst;
   long pitch;
-  int transpose;
   int tiles_x;
   int block_start;
 };
 struct F32Staging {
   std::vector<F32StageCopy> copies;  // dst set by apply
   std::vector<size_t> off;           // floats into the workspace, per copy
-  std::vector<int> a_copy, b_copy;   // per descriptor; b_copy -1: B is read in place
+  std::vector<int> a_copy;           // per descriptor
   size_t total = 0;                  // floats
   int blocks = 0;                    // tiles of the batched copy kernel
   void plan(const F32Desc *descs, int ndesc);

@@ -79,8 +79,9 @@ def _f32_plan_with_config(cfg, gemms):
 
 def test_f32_plan_transposed_a_bit_identical_and_replays():
     """FP32 plans on the transposed-A configuration: one batched kernel stages every A
-    transposed (an A shared by two items once) and every B whose rows are not 16-byte
-    aligned, then one persistent launch runs all items. The kernel takes k in the same pairs
+    transposed (an A shared by two items once), then one persistent launch runs all items,
+    reading B in place (4 bytes per lane, since some rows are not 16-byte aligned). The
+    kernel takes k in the same pairs
     and order as the register-staged configurations, so the results equal configuration 2's
     bit for bit, within the FP32 bound; captured replays (eager work in between) equal eager
     launches."""
@@ -246,3 +247,24 @@ def test_f64_plan_transposed_a_graph_replays():
         assert torch.equal(dCg.view(torch.int64), dCd.view(torch.int64)), rep
     pg.close()
     pe.close()
+
+
+def test_f32_plan_transposed_a_aligned_b_uses_16_byte_loads():
+    """Every B 16-byte aligned with a pitch of 4 floats: the 16-byte B load form of the
+    transposed-A kernel, bit for bit equal to configuration 2."""
+    import torch
+    shapes = [(300, 256, 200), (129, 132, 17), (640, 512, 64)]
+    host = [(O.uniform((m, k), 30 + i).astype(np.float32), O.uniform((k, n), 40 + i).astype(np.float32),
+             O.uniform((m, n), 50 + i).astype(np.float32)) for i, (m, n, k) in enumerate(shapes)]
+    outs = []
+    for cfg in (3, 2):
+        g = [(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(), torch.from_numpy(C0.copy()).cuda())
+             for A, B, C0 in host]
+        plan = _f32_plan_with_config(cfg, g)
+        assert plan.info()["config"] == cfg
+        plan.launch()
+        torch.cuda.synchronize()
+        outs.append([c.cpu().numpy() for _, _, c in g])
+        plan.close()
+    for a, b in zip(*outs):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))

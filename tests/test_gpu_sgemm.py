@@ -49,7 +49,7 @@ def _sgemm(A, B, C0, key6, b_view=None):
 
 
 # large enough (>= 2 tiles of 128 x 128 per CU) for the transposed-A kernel; n = 2052 keeps
-# B as it is (16-byte rows), n = 2049 / 2051 stage it to a pitch of 4 floats
+# B with 16-byte loads, n = 2049 / 2051 (rows not 16-byte aligned) with 4-byte loads
 @pytest.mark.parametrize("m,n,k", [(4100, 2052, 300), (4097, 2049, 1000), (4096, 2051, 17), (4224, 4096, 2)])
 def test_fp32_transposed_a_bit_identical_to_register_staged(m, n, k):
     """The transposed-A FP32 kernel takes k in the same pairs, in the same order, as the
