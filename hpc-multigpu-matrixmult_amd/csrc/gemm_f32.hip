@@ -576,31 +576,24 @@ __global__ __launch_bounds__(256) void f32_transpose_kernel(const float *__restr
 }
 
 // All transposed A copies of a grouped transposed-A launch in one kernel: one workgroup per
-// 32 x 32 tile of a copy's destination, the copy found by binary search over block_start.
-__global__ __launch_bounds__(256) void f32_stage_kernel(const F32StageCopy *__restrict__ copies, int ncopies) {
-  __shared__ float tile[32][33];
+// 64 x 64 tile of a copy's destination, its copy read from a per-tile table.
+__global__ __launch_bounds__(256) void f32_stage_kernel(const F32StageCopy *__restrict__ copies,
+                                                        const int *__restrict__ block_copy) {
+  __shared__ float tile[64][65];
   const int b = blockIdx.x;
-  int lo = 0, hi = ncopies - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (copies[mid].block_start <= b)
-      lo = mid;
-    else
-      hi = mid - 1;
-  }
-  const F32StageCopy c = copies[lo];
+  const F32StageCopy c = copies[block_copy[b]];
   const int lb = b - c.block_start;
-  const int d0 = (lb / c.tiles_x) * 32;  // first destination row (k) of the tile
-  const int e0 = (lb % c.tiles_x) * 32;  // first destination column (m)
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int d0 = (lb / c.tiles_x) * 64;  // first destination row (k) of the tile
+  const int e0 = (lb % c.tiles_x) * 64;  // first destination column (m)
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
 #pragma unroll
-  for (int r = ty; r < 32; r += 8) {
+  for (int r = ty; r < 64; r += 4) {
     const int sr = e0 + r, sc = d0 + tx;
     tile[r][tx] = (sr < c.rows && sc < c.cols) ? c.src[(long)sr * c.ld + sc] : 0.0f;
   }
   __syncthreads();
 #pragma unroll
-  for (int r = ty; r < 32; r += 8) {
+  for (int r = ty; r < 64; r += 4) {
     const int dr = d0 + r, dc = e0 + tx;
     if (dr < c.cols && dc < c.pitch) c.dst[(long)dr * c.pitch + dc] = tile[tx][r];
   }
@@ -743,6 +736,7 @@ bool phpc_f32_at_ok(const F32Desc *descs, int ndesc) {
 void F32Staging::plan(const F32Desc *descs, int ndesc) {
   copies.clear();
   off.clear();
+  block_copy.clear();
   a_copy.assign(ndesc, -1);
   total = 0;
   blocks = 0;
@@ -757,9 +751,11 @@ void F32Staging::plan(const F32Desc *descs, int ndesc) {
     c.rows = rows;
     c.cols = cols;
     c.pitch = ((long)rows + 3) / 4 * 4;
-    c.tiles_x = (int)((c.pitch + 31) / 32);
+    c.tiles_x = (int)((c.pitch + 63) / 64);
     c.block_start = blocks;
-    blocks += c.tiles_x * ((cols + 31) / 32);
+    const int nt = c.tiles_x * ((cols + 63) / 64);
+    blocks += nt;
+    block_copy.insert(block_copy.end(), nt, (int)copies.size());
     copies.push_back(c);
     off.push_back(total);
     const size_t n = (size_t)cols * c.pitch;
@@ -783,10 +779,24 @@ void F32Staging::apply(float *ws, F32Desc *descs, int ndesc) {
   }
 }
 
-int F32Staging::run(const F32StageCopy *copies_dev, hipStream_t stream) const {
+static size_t copies_bytes(size_t n) { return (n * sizeof(F32StageCopy) + 255) / 256 * 256; }
+
+size_t F32Staging::list_bytes() const { return copies_bytes(copies.size()) + block_copy.size() * sizeof(int); }
+
+int F32Staging::upload(void *list_dev) const {
+  if (copies.empty()) return PHPC_OK;
+  char *p = static_cast<char *>(list_dev);
+  PHPC_HIP(hipMemcpy(p, copies.data(), copies.size() * sizeof(F32StageCopy), hipMemcpyHostToDevice));
+  PHPC_HIP(hipMemcpy(p + copies_bytes(copies.size()), block_copy.data(), block_copy.size() * sizeof(int),
+                     hipMemcpyHostToDevice));
+  return PHPC_OK;
+}
+
+int F32Staging::run(const void *list_dev, hipStream_t stream) const {
   if (copies.empty() || blocks <= 0) return PHPC_OK;
-  hipLaunchKernelGGL(f32_stage_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, copies_dev,
-                     (int)copies.size());
+  const char *p = static_cast<const char *>(list_dev);
+  hipLaunchKernelGGL(f32_stage_kernel, dim3((unsigned)blocks), dim3(256), 0, stream,
+                     reinterpret_cast<const F32StageCopy *>(p), reinterpret_cast<const int *>(p + copies_bytes(copies.size())));
   PHPC_HIP(hipGetLastError());
   return PHPC_OK;
 }

@@ -31,7 +31,7 @@ struct phpc_plan {
   F32Staging staging32;
   std::vector<F32Desc> d32_at;
   float *at32_ws = nullptr;
-  F32StageCopy *copies_dev = nullptr;
+  void *copies_dev = nullptr;  // the staging list (F32Staging::list_bytes)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t graph_stream = nullptr;
@@ -77,7 +77,7 @@ int plan_prepare_at32(phpc_plan *p) {
   if (p->at32_ws) return PHPC_OK;
   p->staging32.plan(p->d32.data(), p->count);
   const size_t ws_bytes = p->staging32.total * sizeof(float);
-  const size_t list_bytes = p->staging32.copies.size() * sizeof(F32StageCopy);
+  const size_t list_bytes = p->staging32.list_bytes();
   if (hipMalloc(&p->at32_ws, ws_bytes) != hipSuccess) {
     p->at32_ws = nullptr;
     return phpc_set_error(PHPC_ERR_MEM, "plan FP32 staging workspace allocation failed (%zu bytes)", ws_bytes);
@@ -91,7 +91,7 @@ int plan_prepare_at32(phpc_plan *p) {
   p->d32_at = p->d32;
   p->staging32.apply(p->at32_ws, p->d32_at.data(), p->count);
   if (list_bytes)
-    PHPC_HIP(hipMemcpy(p->copies_dev, p->staging32.copies.data(), list_bytes, hipMemcpyHostToDevice));
+    if (int rc = p->staging32.upload(p->copies_dev)) return rc;
   return PHPC_OK;
 }
 

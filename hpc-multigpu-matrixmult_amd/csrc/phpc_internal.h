@@ -79,7 +79,7 @@ int phpc_f32_pick_plan_config(const struct F32Desc *descs, int ndesc);
 // transposed-A direct-to-LDS kernel (128x128), which reads staged operand copies.
 constexpr int kF32AtConfig = 3;
 // One transposed A copy of a grouped FP32 transposed-A launch, filled by one batched kernel:
-// dst[c][r] = src[r][c] for r < rows, c < cols, zero for rows <= r < pitch. 32 x 32 tiles of
+// dst[c][r] = src[r][c] for r < rows, c < cols, zero for rows <= r < pitch. 64 x 64 tiles of
 // dst, tiles_x across a dst row; block_start is the first tile's index in the batched grid.
 // B is read in place (the kernel loads unaligned B rows 4 bytes per lane).
 struct F32StageCopy {
@@ -95,11 +95,15 @@ struct F32Staging {
   std::vector<F32StageCopy> copies;  // dst set by apply
   std::vector<size_t> off;           // floats into the workspace, per copy
   std::vector<int> a_copy;           // per descriptor
+  std::vector<int> block_copy;       // the copy of each tile of the batched kernel
   size_t total = 0;                  // floats
   int blocks = 0;                    // tiles of the batched copy kernel
   void plan(const F32Desc *descs, int ndesc);
   void apply(float *ws, F32Desc *descs, int ndesc);
-  int run(const F32StageCopy *copies_dev, hipStream_t stream) const;
+  // the device list the kernel reads: the copies (dst set by apply), then block_copy
+  size_t list_bytes() const;
+  int upload(void *list_dev) const;
+  int run(const void *list_dev, hipStream_t stream) const;
 };
 // the descriptors fit the transposed-A kernel's 32-bit buffer offsets once staged
 bool phpc_f32_at_ok(const F32Desc *descs, int ndesc);
