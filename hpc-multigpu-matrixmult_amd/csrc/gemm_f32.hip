@@ -925,7 +925,8 @@ extern "C" int phpc_sgemm_device(const float *dA, long lda, const float *dB, lon
   } else {
     const long work = (long)((m + 127) / 128) * ((n + 127) / 128);
     const int cfg = work >= 2L * phpc_device().cus ? 2 : 0;
-    if (cfg == 2 && g_f32_at && (m + 3L) / 4 * 4 / 32 < 65535 && k < 65535)
+    // the transpose's grid.y is the staged pitch / 32
+    if (cfg == 2 && g_f32_at && (m + 3L) / 4 * 4 / 32 < 65535)
       rc = f32_launch_transposed(d, (hipStream_t)stream);
     else {
       const int tiles = phpc_f32_plan_tiles(&d, 1, cfg);
