@@ -260,3 +260,28 @@ def test_special_values_match_reference_kernel(cfg):
     C = C0.copy()
     phpc.gemm_cuda(A, B, C, grid_width=4, grid_height=4, block_width=16)
     assert same_bits_or_both_nan(C, want)
+
+
+@pytest.mark.parametrize("cfg", [0, 12])
+@pytest.mark.parametrize("m,k,n,panel", [(300, 1000, 260, 333), (129, 70, 257, 16), (513, 2050, 130, 1024),
+                                         (64, 64, 64, 63)])
+def test_panel_grouped_configs_on_ragged_panels(cfg, m, k, n, panel):
+    """Panel-grouped work through the device entry point, register-staged (0) and
+    transposed-A (12): C = ((C0 + s_0) + s_1) + ..., each s_t an FMA chain from zero over
+    one panel, bit-exact against per-panel calls of the reference kernel's emulation, on
+    ragged shapes and panels that do not divide k."""
+    import torch
+    A, B, C0 = O.uniform((m, k), 81), O.uniform((k, n), 82), O.uniform((m, n), 83)
+    want = C0.copy()
+    for k0 in range(0, k, panel):
+        k1 = min(k, k0 + panel)
+        s = O.kernel_emul(np.ascontiguousarray(A[:, k0:k1]), np.ascontiguousarray(B[k0:k1]), np.zeros((m, n)))
+        want = want + s
+    dC = torch.from_numpy(C0.copy()).cuda()
+    phpc.set_launch_policy(0, cfg, 0)
+    try:
+        phpc.dgemm_device(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(), dC, panel=panel)
+        torch.cuda.synchronize()
+    finally:
+        phpc.set_launch_policy(0, -1, 0)
+    assert np.array_equal(bits(dC.cpu().numpy()), bits(want))
