@@ -89,6 +89,28 @@ def test_fp32_transposed_a_on_offset_views():
     assert np.all(np.abs(outs[0].astype(np.float64) - ref) <= bound(A, B, C0))
 
 
+def test_fp32_transposed_a_special_values():
+    """Infinities, NaNs and signed zeros through the transposed-A FP32 kernel give the
+    register-staged kernel's results: NaNs at the same places, every other entry bit for bit
+    (the same k pairs in the same order)."""
+    m, n, k = 4100, 2052, 64
+    A = O.uniform((m, k), 61).astype(np.float32)
+    B = O.uniform((k, n), 62).astype(np.float32)
+    C0 = O.uniform((m, n), 63).astype(np.float32)
+    A[5, 3] = np.inf
+    A[6, 4] = -np.inf
+    B[3, 9] = 0.0                # C[5, 9]: inf * 0 = NaN
+    B[4, 10] = np.nan
+    A[7, :] = -0.0
+    C0[7, :] = -0.0
+    C0[8, 8] = np.inf
+    got, want = _sgemm(A, B, C0, 1), _sgemm(A, B, C0, 0)
+    gn, wn = np.isnan(got), np.isnan(want)
+    assert np.array_equal(gn, wn) and gn[5, 9] and gn[4, 10]
+    assert np.array_equal(np.where(gn, 0, got).view(np.uint32), np.where(wn, 0, want).view(np.uint32))
+    assert np.isinf(got[8, 8]) and np.isinf(got[5, 0]) and not np.signbit(got[7, 0])
+
+
 def test_bf16x3_wide_range_inputs():
     """Entries spread over 2^-40 .. 2^40: each slice keeps its own exponent."""
     import torch

@@ -75,6 +75,28 @@ def test_split_auto_slices_meet_fp64_target(gen, method):
     assert info.error_estimate <= 1e-14
 
 
+@pytest.mark.parametrize("method", [0, 1])
+def test_split_zero_rows_and_columns(method):
+    """All-zero rows of A and columns of B (ratio 0, left out of the modulus bound) leave
+    their rows and columns of C exactly at C0; the rest meets the FP64 target, including a
+    row whose entries are all far below the others' scale."""
+    import torch
+    m, k, n = 200, 300, 150
+    A, B, C0 = O.wide((m, k), 31), O.wide((k, n), 32), O.uniform((m, n), 33)
+    A[[0, 7, 199], :] = 0.0
+    B[:, [0, 5, 149]] = 0.0
+    A[50, :] *= 2.0 ** -600
+    dC = dev(C0.copy())
+    info = phpc.dgemm_split_device(dev(A), dev(B), dC, slices=0, target=1e-14, method=method)
+    torch.cuda.synchronize()
+    got = dC.cpu().numpy()
+    assert np.array_equal(got[[0, 7, 199], :], C0[[0, 7, 199], :])
+    assert np.array_equal(got[:, [0, 5, 149]], C0[:, [0, 5, 149]])
+    exact = C0 + O.gemm_dd(A, B)
+    mag = np.abs(C0) + np.abs(A) @ np.abs(B)
+    assert np.all(np.abs(got - exact) <= 1e-14 * mag), (info.slices, info.int_bits)
+
+
 def test_split_rejects_non_finite():
     import torch
     A = np.ones((8, 8))
