@@ -587,13 +587,15 @@ __global__ __launch_bounds__(256) void f32_stage_kernel(const F32StageCopy *__re
   const int e0 = (lb % c.tiles_x) * 64;  // first destination column (m)
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
 #pragma unroll
-  for (int r = ty; r < 64; r += 4) {
+  for (int i = 0; i < 16; ++i) {
+    const int r = ty + 4 * i;
     const int sr = e0 + r, sc = d0 + tx;
     tile[r][tx] = (sr < c.rows && sc < c.cols) ? c.src[(long)sr * c.ld + sc] : 0.0f;
   }
   __syncthreads();
 #pragma unroll
-  for (int r = ty; r < 64; r += 4) {
+  for (int i = 0; i < 16; ++i) {
+    const int r = ty + 4 * i;
     const int dr = d0 + r, dc = e0 + tx;
     if (dr < c.cols && dc < c.pitch) c.dst[(long)dr * c.pitch + dc] = tile[tx][r];
   }
