@@ -38,33 +38,79 @@ def load_phpc():
     return mod
 
 
-def cpu_baseline():
-    """The reference's CPU path (src/iterative.c), timed on one host core at N = 1024."""
-    ref = ROOT / "oracle" / "_ref" / "iterative.out"
-    port = ROOT / "oracle" / "_build" / "iterative_O0.out"
-    exe, kind = (ref, "reference") if ref.exists() else (port, "port")
-    if not exe.exists():
+def _cpu_model():
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(n=1024, seed=1, runs=3):
+    """The reference's CPU path (src/iterative.c:8-13) on the first workload's inputs: N = 1024
+    FP64, A and B uniform[-1,1) from seeds 1 and 2, timed on one pinned host core. The -O0
+    build is the reference's own flags (Makefile:9) and gives `value`; -O2 is an extra line.
+    C is checked bit for bit against the oracle's restatement of the same loop and against
+    the double-double product."""
+    import tempfile
+
+    import numpy as np
+
+    sys.path.insert(0, str(ROOT))
+    import oracle as O
+
+    ref = {lvl: ROOT / "oracle" / "_ref" / f"iterative_ref_{lvl}.out" for lvl in ("O0", "O2")}
+    port = {lvl: ROOT / "oracle" / "_build" / f"iterative_{lvl}.out" for lvl in ("O0", "O2")}
+    kind = "reference" if all(p.exists() for p in ref.values()) else "port"
+    exes = ref if kind == "reference" else port
+    if not all(p.exists() for p in exes.values()):
         return None
-    n, runs = 1024, 3
-    cmd = [str(exe), str(n)]
-    if shutil.which("taskset"):
-        cmd = ["taskset", "-c", "0"] + cmd
-    secs = []
-    for _ in range(runs):
-        out = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
-        if out.returncode != 0:
-            return None
-        secs.append(float(out.stdout.strip().split(",")[1]))
-    t = statistics.median(secs)
-    tflops = 2.0 * n ** 3 / t / 1e12
+    pin = ["taskset", "-c", "0"] if shutil.which("taskset") else []
+    lines, c_out = {}, None
+    with tempfile.TemporaryDirectory() as td:
+        for lvl, exe in exes.items():
+            secs = []
+            for r in range(runs):
+                out_file = os.path.join(td, f"c_{lvl}.bin")
+                cmd = [str(exe), str(n), str(seed), out_file] if kind == "reference" else [str(exe), str(n), "u"]
+                res = subprocess.run(pin + cmd, capture_output=True, text=True, timeout=600)
+                if res.returncode != 0:
+                    return None
+                secs.append(float(res.stdout.strip().split(",")[1]))
+                if kind == "reference" and r == 0:
+                    c = np.fromfile(out_file, dtype=np.float64).reshape(n, n)
+                    if c_out is None:
+                        c_out = c
+                    elif not np.array_equal(c.view(np.uint64), c_out.view(np.uint64)):
+                        c_out = None
+                        lines["mismatch"] = "O0 and O2 builds differ"
+            t = statistics.median(secs)
+            lines[lvl] = {"seconds_median": round(t, 4), "seconds_runs": [round(x, 4) for x in secs],
+                          "tflops": 2.0 * n ** 3 / t / 1e12, "gflops": round(2.0 * n ** 3 / t / 1e9, 4),
+                          "build": f"gcc -{lvl} -Wall (reference Makefile:9 flags{'' if lvl == 'O0' else ' + -O2'})"}
+    checked = None
+    if c_out is not None:
+        A, B = O.uniform((n, n), seed), O.uniform((n, n), seed + 1)
+        want = O.iterative(A, B, np.zeros((n, n)))
+        bit_exact = bool(np.array_equal(c_out.view(np.uint64), want.view(np.uint64)))
+        cw, _ = O.err_full(c_out, O.gemm_dd(A, B), A, B)
+        checked = {"bit_exact_vs_oracle_iterative": bit_exact, "err_cw_vs_double_double": cw,
+                   "bound_n_u": n * 2.0 ** -53}
+    t0 = lines["O0"]["seconds_median"]
     return {
-        "value": tflops,
+        "value": lines["O0"]["tflops"],
         "unit": "TFLOP/s",
         "cores": 1,
         "kind": kind,
-        "sample": (f"{exe.relative_to(ROOT)} n={n} (ramp inputs, gcc -O0 as the reference Makefile), "
-                   f"median of {runs} runs = {t:.3f} s on 1 pinned core of {os.cpu_count()}; "
-                   f"N=16384 would take ~{t * 16 ** 3 / 3600:.1f} h (O(N^3) extrapolation)"),
+        "sample": (f"src/iterative.c loop at N={n}, A/B uniform[-1,1) seeds {seed}/{seed + 1}, "
+                   f"gcc -O0 as the reference Makefile, median of {runs} runs = {t0:.3f} s on 1 pinned core "
+                   f"({os.cpu_count()} visible, {_cpu_model()}); N=16384 would take "
+                   f"~{t0 * 16 ** 3 / 3600:.1f} h (O(N^3) extrapolation)"),
+        "lines": lines,
+        "checked": checked,
+        "host": {"nproc": os.cpu_count(), "cpu_model": _cpu_model(), "pinned_core": 0 if pin else None},
     }
 
 
@@ -194,7 +240,7 @@ def grouped_bench(phpc, seed=21):
     return out
 
 
-def split_sweep(phpc, sizes=(4096, 8192), seed=31):
+def split_sweep(phpc, sizes=(4096, 8192, 16384), seed=31):
     """Split-precision FP64 GEMM at each slice / modulus count: error against the
     double-double product on sampled entries and FP64-equivalent TFLOP/s."""
     import numpy as np
@@ -365,6 +411,7 @@ def serving_bench(phpc, n_req, seed=11, slots=3, batch_max=32, split_min=4096, l
         sat_rate = n_req / t_sat
         _, peak = srv.memory()
         out = {
+            "default_policy": "fifo",
             "requests": n_req, "slots": slots, "batch_max": batch_max, "budget_bytes": 200 * 10 ** 9,
             "fp64_split_min_dim": split_min,
             "mix": "sizes 60% [256,1024], 30% [1024,4096], 10% [4096,8192]; 50% FP64-level, 50% FP32-level",
@@ -407,6 +454,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--only", choices=["summa", "split", "fp32acc", "grouped", "sweep", "serve"],
                     help="run one section alone and print its JSON (for profiler runs)")
+    ap.add_argument("--mode", choices=["chained", "fused"], default="chained",
+                    help="SUMMA schedule of --only summa")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy three times (counter calibration)")
     args = ap.parse_args()
@@ -444,11 +493,12 @@ def main():
                 if i == 1:
                     torch.cuda.synchronize()
                     t0 = time.perf_counter()
-                phpc.check(phpc.lib().phpc_summa_device(p, q, args.panel, phpc.SUMMA_CHAINED, phpc.dptr(A),
+                mode = phpc.SUMMA_CHAINED if args.mode == "chained" else phpc.SUMMA_FUSED
+                phpc.check(phpc.lib().phpc_summa_device(p, q, args.panel, mode, phpc.dptr(A),
                                                          phpc.dptr(B), phpc.dptr(C), args.n, 0, sp))
             torch.cuda.synchronize()
             dt = (time.perf_counter() - t0) / max(1, args.steps)
-            res = {"mode": "chained", "n": args.n, "grid": args.grid, "panel": args.panel,
+            res = {"mode": args.mode, "n": args.n, "grid": args.grid, "panel": args.panel,
                    "tflops": round(2.0 * args.n ** 3 / dt / 1e12, 2)}
         elif args.only == "split":
             # the split CRT GEMM at N (uniform inputs), one warm-up and steps timed calls
@@ -493,46 +543,49 @@ def main():
             phpc.check(rc)
 
     def timed(mode, steps, warmup):
+        """Whole-step time (events around the K steps on the launch stream, and the wall
+        clock), and the average duration of the GEMM kernel launch alone (the library's
+        events around that launch on the same stream)."""
         for _ in range(warmup):
             run_mode(mode)
         torch.cuda.synchronize()
-        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
         t_start = torch.cuda.Event(enable_timing=True)
         t_end = torch.cuda.Event(enable_timing=True)
+        phpc.kernel_timing(True)
         torch.cuda.synchronize()
         w0 = time.perf_counter()
         t_start.record(stream)
         for i in range(steps):
-            ev[i][0].record(stream)
             run_mode(mode)
-            ev[i][1].record(stream)
         t_end.record(stream)
         torch.cuda.synchronize()
         wall_s = time.perf_counter() - w0
-        return t_start.elapsed_time(t_end) / steps, statistics.mean(a.elapsed_time(b) for a, b in ev), wall_s
+        k_ms, k_n = phpc.kernel_timing_read()
+        phpc.kernel_timing(False)
+        return t_start.elapsed_time(t_end) / steps, k_ms / max(1, k_n), k_n, wall_s
 
     flops = 2.0 * N ** 3
     # primary: CHAINED schedule (one FMA chain per output over the 64 panels)
-    ms_per_step, kernel_ms, wall = timed(phpc.SUMMA_CHAINED, args.steps, args.warmup)
+    ms_per_step, kernel_ms, k_launches, wall = timed(phpc.SUMMA_CHAINED, args.steps, args.warmup)
     value = flops / (ms_per_step * 1e-3) / 1e12
     achieved = flops / (kernel_ms * 1e-3) / 1e12
     # the reference's per-panel grouping (C rounded after every panel), same launch shape
-    exact_ms, _, _ = timed(phpc.SUMMA_FUSED, max(2, args.steps // 2), 1)
-    step = lambda: run_mode(phpc.SUMMA_CHAINED)  # noqa: E731
+    exact_ms, exact_kernel_ms, _, _ = timed(phpc.SUMMA_FUSED, max(2, args.steps // 2), 1)
+    exact_tf = flops / (exact_ms * 1e-3) / 1e12
+    exact_kernel_tf = flops / (exact_kernel_ms * 1e-3) / 1e12
 
     # accuracy of the timed path: one clean pass, sampled double-double check on the device
     C.zero_()
-    step()
+    run_mode(phpc.SUMMA_CHAINED)
     torch.cuda.synchronize()
     rng = np.random.default_rng(3)
     rows = rng.integers(0, N, 4096)
     cols = rng.integers(0, N, 4096)
     err_cw, err_nw = phpc.check_sampled(A, B, C, rows, cols)
 
-    extra = {}
+    detail = {}
     # split-precision FP64 (CRT / Ozaki scheme II on int8 MFMA), same N, both input sets
-    split = {"method": "crt (Ozaki scheme II): int8 residue GEMMs, Garner reconstruction",
-             "target_err_cw": 1e-14}
+    split = {}
     for dist, tag in ((phpc.DIST_UNIFORM, "uniform"), (phpc.DIST_WIDE, "wide_exponent")):
         phpc.fill_device(A, dist, 1)
         phpc.fill_device(B, dist, 2)
@@ -545,11 +598,14 @@ def main():
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         reps = max(2, args.steps // 2)
+        phpc.kernel_timing(True)
         e0.record(stream)
         for _ in range(reps):
             run_split()
         e1.record(stream)
         torch.cuda.synchronize()
+        lvl_ms, lvl_n = phpc.kernel_timing_read()
+        phpc.kernel_timing(False)
         t_ms = e0.elapsed_time(e1) / reps
         C.zero_()
         run_split()
@@ -557,13 +613,17 @@ def main():
         cw, nw = phpc.check_sampled(A, B, C, rows, cols)
         i = info[0]
         tf = flops / (t_ms * 1e-3) / 1e12
+        kp = (N + 63) // 64 * 64
+        lvl_avg = lvl_ms / max(1, lvl_n)
         split[tag] = {"tflops_fp64_equiv": round(tf, 3), "ms": round(t_ms, 3), "vs_plain": round(tf / value, 3),
                       "moduli": i.slices, "int_bits": i.int_bits, "error_estimate": i.error_estimate,
                       "err_cw_max_sampled": cw, "err_nw_sampled": nw,
-                      "int8_mfma_util": round(i.products * flops / (t_ms * 1e-3) / 5.0e15, 4)}
-    extra["split"] = split
+                      "int8_kernel_ms_avg": round(lvl_avg, 4), "int8_launches": lvl_n,
+                      "int8_kernel_tops": round(2.0 * N * N * kp / (lvl_avg * 1e-3) / 1e12, 2),
+                      "int8_share_of_time": round(lvl_ms / reps / t_ms, 4)}
     phpc.fill_device(A, phpc.DIST_UNIFORM, 1)
     phpc.fill_device(B, phpc.DIST_UNIFORM, 2)
+    vendor = {}
     if not args.no_vendor:
         # rocBLAS comparator on the same SUMMA schedule and as one plain DGEMM
         C.zero_()
@@ -574,7 +634,7 @@ def main():
         e1.record(stream)
         torch.cuda.synchronize()
         if rc == 0:
-            extra["vendor_summa_tflops"] = round(flops / (e0.elapsed_time(e1) * 1e-3) / 1e12, 3)
+            vendor["summa_tflops"] = round(flops / (e0.elapsed_time(e1) * 1e-3) / 1e12, 3)
         C.zero_()
         phpc.dgemm_vendor_device(A, B, C)
         torch.cuda.synchronize()
@@ -582,19 +642,22 @@ def main():
         phpc.dgemm_vendor_device(A, B, C)
         e1.record(stream)
         torch.cuda.synchronize()
-        extra["vendor_dgemm_tflops"] = round(flops / (e0.elapsed_time(e1) * 1e-3) / 1e12, 3)
+        vendor["dgemm_tflops"] = round(flops / (e0.elapsed_time(e1) * 1e-3) / 1e12, 3)
 
     del A, B, C
     torch.cuda.empty_cache()
     if not args.no_grouped:
-        extra["grouped"] = grouped_bench(phpc)
+        detail["grouped"] = grouped_bench(phpc)
     if not args.no_sweep:
-        extra["split_sweep"] = split_sweep(phpc)
-        extra["fp32_accurate"] = fp32_accurate_bench(phpc)
+        detail["split_sweep"] = split_sweep(phpc)
+        detail["fp32_accurate"] = fp32_accurate_bench(phpc)
     if not args.no_serve:
-        extra["serving"] = serving_bench(phpc, args.serve_requests)
+        detail["serving"] = serving_bench(phpc, args.serve_requests)
+    # the full detail on a line of its own, before the result line
+    print("bench-detail " + json.dumps(detail), flush=True)
 
     workload = f"summa_fp64_n{N}_grid{p}x{q}_panel{panel}"
+    uni, wide = split["uniform"], split["wide_exponent"]
     out = {
         "metric": "FP64-accurate TFLOP/s (split vs plain) at N=16384; p50/p99 req latency",
         "value": round(value, 3),
@@ -618,8 +681,6 @@ def main():
                     "output over the 64 panels",
             "flops_per_step": flops,
         },
-        "accuracy": {"err_cw_max_sampled": err_cw, "err_nw_sampled": err_nw, "samples": 4096,
-                     "reference": "double-double on the device"},
         "roofline": {
             "bound": "mfma",
             "achieved": round(achieved, 3),
@@ -627,14 +688,84 @@ def main():
             "unit": "TFLOP/s",
             "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
             "traffic": traffic_from_profiles(workload),
+            "traffic_source": "profiles/traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE of this kernel, "
+                              "scaled by a 1 GiB copy), not measured in this run",
             "kernel": "f64_at_128x128 (transposed-A, direct-to-LDS, 2 workgroups per CU)",
             "kernel_ms_avg": round(kernel_ms, 3),
-            "timed_unit": "one SUMMA pass: 4 A row-block transposes (~1% of the pass) + the GEMM launch",
+            "kernel_launches_timed": k_launches,
+            "pass_ms_avg": round(ms_per_step, 3),
+            "timed_unit": "achieved = 2N^3 / kernel_ms_avg (HIP events around the GEMM launch alone, on its "
+                          "stream); a step (pass) adds the 4 A row-block transposes",
         },
+        "accuracy": {"err_cw_max_sampled": err_cw, "err_nw_sampled": err_nw, "samples": 4096,
+                     "reference": "double-double on the device"},
+        # the metric's parts: split vs plain at N, and the serving tail
+        "split_tflops": {"uniform": uni["tflops_fp64_equiv"], "wide_exponent": wide["tflops_fp64_equiv"]},
+        "split_vs_plain": {"uniform": uni["vs_plain"], "wide_exponent": wide["vs_plain"]},
+        "split_err_cw": {"uniform": uni["err_cw_max_sampled"], "wide_exponent": wide["err_cw_max_sampled"],
+                         "target": 1e-14},
+        "split_moduli": {"uniform": uni["moduli"], "wide_exponent": wide["moduli"]},
+        "split_roofline": {
+            "bound": "mfma",
+            "kernel": "i8_level_256x256_w16_k128 (one int8 residue GEMM per modulus)",
+            "achieved": uni["int8_kernel_tops"],
+            "peak": 5000.0,
+            "unit": "TOP/s",
+            "frac": round(uni["int8_kernel_tops"] / 5000.0, 4),
+            "kernel_ms_avg": uni["int8_kernel_ms_avg"],
+            "ops_per_launch": 2.0 * N * N * ((N + 63) // 64 * 64),
+            "share_of_split_time": uni["int8_share_of_time"],
+        },
+        "split": split,
+        "summa_panel_exact": {
+            "tflops": round(exact_tf, 3),
+            "roofline": {"bound": "mfma", "achieved": round(exact_kernel_tf, 3), "peak": FP64_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": round(exact_kernel_tf / FP64_PEAK_TFLOPS, 4),
+                         "kernel": "f64_at_128x128_w8_mp (per-panel sums added to the C carried in registers)",
+                         "kernel_ms_avg": round(exact_kernel_ms, 3), "pass_ms_avg": round(exact_ms, 3)},
+            "arithmetic": "bit-exact to the reference's per-panel grouping (src/phpc_summa.c:63-95)",
+        },
+        "vendor": vendor,
         "wall_s_timed": round(wall, 3),
-        "summa_panel_exact_tflops": round(flops / (exact_ms * 1e-3) / 1e12, 3),
     }
-    out.update(extra)
+    if "serving" in detail:
+        srv = detail["serving"]
+        default = srv.get("default_policy", "fifo")
+        at = {tag: {str(r["load"]): r["p99_vs_isolated_p99"] for r in runs} for tag, runs in srv["poisson"].items()}
+        r70 = next(r for r in srv["poisson"][default] if abs(r["load"] - 0.7) < 1e-9)
+        out["serving_p99_ratio_at_0.7"] = r70["p99_vs_isolated_p99"]
+        out["serving"] = {
+            "default_policy": default,
+            "p99_vs_isolated_p99": at,
+            "at_0.7": {k: r70[k] for k in ("p50_ms", "p99_ms", "rate_req_per_s", "tflops", "mean_batch",
+                                            "slowdown_p99")},
+            "isolated_ms": srv["isolated_ms"],
+            "saturated": {k: srv["saturated"][k] for k in ("req_per_s", "tflops", "mean_batch")},
+            "requests": srv["requests"], "budget_bytes": srv["budget_bytes"],
+        }
+    if "grouped" in detail:
+        g = detail["grouped"]
+        out["grouped"] = {
+            "fp64_n10000_grid3x3": {"tflops": g["fp64_n10000_grid3x3"]["tflops"],
+                                    "frac_fp64_peak": round(g["fp64_n10000_grid3x3"]["tflops"] / FP64_PEAK_TFLOPS, 4),
+                                    "err_cw": g["fp64_n10000_grid3x3"]["err_cw_max_sampled"]},
+            "fp32_512_gemms": {"tflops": g["fp32_512_gemms"]["tflops"],
+                               "frac_fp32_peak": g["fp32_512_gemms"]["fp32_mfma_util"],
+                               "err_cw": g["fp32_512_gemms"]["err_cw_max_sampled"]},
+        }
+    if "fp32_accurate" in detail:
+        out["fp32_accurate"] = {str(r["N"]): {"fp32_mfma_tflops": r["fp32_mfma"]["tflops"],
+                                              "bf16x3_tflops": r["bf16x3"]["tflops"],
+                                              "bf16x3_err_cw": r["bf16x3"]["err_cw_max_sampled"]}
+                                for r in detail["fp32_accurate"]}
+    if "split_sweep" in detail:
+        # error against split count at each N (CRT moduli / slices): err_cw and TFLOP/s
+        sw = {}
+        for r in detail["split_sweep"]:
+            key = f'{r["N"]}_{r["inputs"][:4]}_{r["method"]}'
+            sw.setdefault(key, []).append([r["count"], float(f'{r["err_cw_max_sampled"]:.2e}'),
+                                           round(r["tflops_fp64_equiv"], 1)])
+        out["split_sweep"] = {"columns": ["count", "err_cw", "tflops"], "rows": sw}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

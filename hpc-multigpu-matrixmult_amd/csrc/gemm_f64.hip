@@ -1068,8 +1068,10 @@ int launch_grouped(const GemmDesc *descs_dev, const GemmDesc *descs_host, int nd
     if (G > total) G = total;
   }
   if (G < 1) G = 1;
+  const int kt = phpc_ktime_begin(stream);
   hipLaunchKernelGGL(ce.fn[mp][vec2], dim3(G), dim3(kernel_threads(config_id, mp, vec2)), ce.lds_bytes[vec2],
                      stream, b);
+  phpc_ktime_end(kt, stream);
   PHPC_HIP(hipGetLastError());
   return PHPC_OK;
 }

@@ -210,6 +210,10 @@ int phpc_pack_launch(const double *src, long lds, double *dst, long ldd, int row
 
 // Scratch device buffer owned by the library (grows; stream-ordered use only).
 void *phpc_scratch(size_t bytes, int slot);
+// Kernel event timing (phpc_kernel_timing): begin returns a token (-1 when timing is off
+// or the stream is capturing); end records the closing event for that token.
+int phpc_ktime_begin(hipStream_t stream);
+void phpc_ktime_end(int token, hipStream_t stream);
 
 // A set of growable device buffers for one caller at a time (the split path needs four).
 // Growing synchronises `stream` before the old buffer is freed.

@@ -98,6 +98,80 @@ extern "C" int phpc_init(void) {
 const DeviceInfo &phpc_device() { return g_dev; }
 LaunchPolicy &phpc_policy() { return g_policy; }
 
+// ---------------------------------------------------------------------------
+// Event timing of the main GEMM kernel launches (measurement only). While enabled,
+// each FP64 tile launch and each split-precision level launch is bracketed by a pair
+// of HIP events on its own stream; reading syncs on them and returns the summed
+// duration. Launches on a stream that is being captured into a graph are not timed.
+// ---------------------------------------------------------------------------
+namespace {
+std::mutex g_kt_mu;
+bool g_kt_on = false;
+std::vector<std::pair<hipEvent_t, hipEvent_t>> g_kt_pairs;  // recorded since the last read
+std::vector<hipEvent_t> g_kt_pool;                          // idle events, reused
+hipEvent_t kt_event() {
+  if (!g_kt_pool.empty()) {
+    hipEvent_t e = g_kt_pool.back();
+    g_kt_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  return hipEventCreate(&e) == hipSuccess ? e : nullptr;
+}
+}  // namespace
+
+int phpc_ktime_begin(hipStream_t stream) {
+  std::lock_guard<std::mutex> lk(g_kt_mu);
+  if (!g_kt_on) return -1;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return -1;
+  hipEvent_t a = kt_event(), b = kt_event();
+  if (!a || !b || hipEventRecord(a, stream) != hipSuccess) {
+    if (a) g_kt_pool.push_back(a);
+    if (b) g_kt_pool.push_back(b);
+    return -1;
+  }
+  g_kt_pairs.emplace_back(a, b);
+  return (int)g_kt_pairs.size() - 1;
+}
+
+void phpc_ktime_end(int token, hipStream_t stream) {
+  if (token < 0) return;
+  std::lock_guard<std::mutex> lk(g_kt_mu);
+  if (token < (int)g_kt_pairs.size()) (void)hipEventRecord(g_kt_pairs[token].second, stream);
+}
+
+extern "C" int phpc_kernel_timing(int on) {
+  std::lock_guard<std::mutex> lk(g_kt_mu);
+  for (auto &pr : g_kt_pairs) {
+    (void)hipEventSynchronize(pr.second);
+    g_kt_pool.push_back(pr.first);
+    g_kt_pool.push_back(pr.second);
+  }
+  g_kt_pairs.clear();
+  g_kt_on = on != 0;
+  return phpc_set_error(PHPC_OK, nullptr);
+}
+
+extern "C" int phpc_kernel_timing_read(double *ms_total, int *launches) {
+  std::lock_guard<std::mutex> lk(g_kt_mu);
+  double total = 0.0;
+  int count = 0, rc = PHPC_OK;
+  for (auto &pr : g_kt_pairs) {
+    float ms = 0.0f;
+    if (hipEventSynchronize(pr.second) != hipSuccess || hipEventElapsedTime(&ms, pr.first, pr.second) != hipSuccess)
+      rc = PHPC_ERR_HIP;
+    total += ms;
+    ++count;
+    g_kt_pool.push_back(pr.first);
+    g_kt_pool.push_back(pr.second);
+  }
+  g_kt_pairs.clear();
+  if (ms_total) *ms_total = total;
+  if (launches) *launches = count;
+  return phpc_set_error(rc, rc ? "kernel timing events failed" : nullptr);
+}
+
 void *phpc_scratch(size_t bytes, int slot) {
   std::lock_guard<std::mutex> lk(g_scratch_mu);
   ScratchSlot &s = g_scratch[slot];
@@ -481,18 +555,29 @@ extern "C" int phpc_check_sampled(const double *dA, long lda, const double *dB, 
   hipStream_t s = (hipStream_t)stream;
   int *didx = nullptr;
   double *dout = nullptr;
-  PHPC_HIP(hipMalloc(&didx, sizeof(int) * 2 * count));
-  PHPC_HIP(hipMalloc(&dout, sizeof(double) * 3 * count));
-  PHPC_HIP(hipMemcpyAsync(didx, rows, sizeof(int) * count, hipMemcpyHostToDevice, s));
-  PHPC_HIP(hipMemcpyAsync(didx + count, cols, sizeof(int) * count, hipMemcpyHostToDevice, s));
+  std::vector<double> h(3 * (size_t)count);
+  // every exit after the first allocation frees both buffers (the stream is drained
+  // first, so no launched kernel still reads or writes them)
+  auto fail = [&](hipError_t e, const char *what) {
+    (void)hipStreamSynchronize(s);
+    if (didx) (void)hipFree(didx);
+    if (dout) (void)hipFree(dout);
+    return phpc_set_error(PHPC_ERR_HIP, "check_sampled: %s failed: %s", what, hipGetErrorString(e));
+  };
+  hipError_t e;
+  if ((e = hipMalloc(&didx, sizeof(int) * 2 * count)) != hipSuccess) return fail(e, "hipMalloc");
+  if ((e = hipMalloc(&dout, sizeof(double) * 3 * count)) != hipSuccess) return fail(e, "hipMalloc");
+  if ((e = hipMemcpyAsync(didx, rows, sizeof(int) * count, hipMemcpyHostToDevice, s)) != hipSuccess ||
+      (e = hipMemcpyAsync(didx + count, cols, sizeof(int) * count, hipMemcpyHostToDevice, s)) != hipSuccess)
+    return fail(e, "sample copy");
   hipLaunchKernelGGL(check_kernel, dim3(count), dim3(64), 0, s, dA, lda, dB, ldb, dC0, ldc0, dC, ldc, k, didx,
                      didx + count, dout, dout + count, dout + 2 * count);
-  PHPC_HIP(hipGetLastError());
-  std::vector<double> h(3 * (size_t)count);
-  PHPC_HIP(hipMemcpyAsync(h.data(), dout, sizeof(double) * 3 * count, hipMemcpyDeviceToHost, s));
-  PHPC_HIP(hipStreamSynchronize(s));
-  PHPC_HIP(hipFree(didx));
-  PHPC_HIP(hipFree(dout));
+  if ((e = hipGetLastError()) != hipSuccess) return fail(e, "check_kernel launch");
+  if ((e = hipMemcpyAsync(h.data(), dout, sizeof(double) * 3 * count, hipMemcpyDeviceToHost, s)) != hipSuccess)
+    return fail(e, "result copy");
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return fail(e, "hipStreamSynchronize");
+  (void)hipFree(didx);
+  (void)hipFree(dout);
   double cw = 0, md = 0, mr = 0;
   for (int e = 0; e < count; ++e) {
     if (h[e] > cw || std::isnan(h[e])) cw = std::isnan(h[e]) ? INFINITY : h[e];

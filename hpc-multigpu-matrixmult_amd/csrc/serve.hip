@@ -95,6 +95,7 @@ struct phpc_server {
   size_t reserved = 0;  // per-slot workspaces
   size_t in_use = 0, peak = 0;
   int batch_max = 16;
+  int max_dim = 0;       // the per-slot workspaces are sized for m, n, k up to this
   int split_min = 4096;  // FP64 class: split CRT GEMM when min(m, n, k) >= split_min
   int policy = PHPC_SCHED_FIFO;
   double aging = 0.0;  // shortest-first: priority = est - wait / aging (0: no aging)
@@ -342,6 +343,7 @@ extern "C" int phpc_server_create(size_t budget_bytes, int slots, int max_dim, i
     return phpc_set_error(PHPC_ERR_ARG, "bad server arguments");
   phpc_server *srv = new phpc_server();
   srv->batch_max = batch_max;
+  srv->max_dim = max_dim;
   // per-slot split workspace for max_dim^3 at up to 20 moduli: two int8 operand copies,
   // 20 residue planes, metadata
   const size_t kp = ((size_t)max_dim + 63) / 64 * 64;
@@ -390,6 +392,11 @@ extern "C" int phpc_server_submit(phpc_server_t srv, const phpc_request *req, ui
       req->dist < 0 || req->dist > 2)
     return phpc_set_error(PHPC_ERR_ARG, "bad request");
   const phpc_request &r = *req;
+  // the per-slot workspaces were allocated once, for dimensions up to max_dim; a wider
+  // request would grow them on the request path, outside the budget
+  if (std::max(r.m, std::max(r.n, r.k)) > srv->max_dim)
+    return phpc_set_error(PHPC_ERR_BUDGET, "request %d x %d x %d exceeds the server's max_dim %d", r.m, r.k, r.n,
+                          srv->max_dim);
   Req *q = new Req();
   q->r = r;
   const size_t e = r.accuracy == PHPC_ACC_FP64 ? 8 : 4;

@@ -43,6 +43,9 @@ constexpr int kMaxSlices = 12;
 constexpr long kMaxExactTerms = 131072;  // products per int32 accumulation (127^2 * 2^17 < 2^31)
 // residues reach -128: 128^2 * 131008 < 2^31 (131008 = 2^17 - 64, a whole number of slabs)
 constexpr long kMaxExactTermsCrt = 131008;
+// Widest integer operand of the CRT scheme: residues_of cuts |x| < 2^72 into nine bytes,
+// and the scaled operands satisfy |x| <= 2^p.
+constexpr int kMaxIntBits = 71;
 
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ long uni(long v) {
@@ -582,7 +585,9 @@ int launch_with(const I8Config &c, int occ, const LevelArgs &a, hipStream_t s) {
   long G = (long)occ * phpc_device().cus;
   if (G > tiles) G = tiles;
   if (G < 1) G = 1;
+  const int kt = phpc_ktime_begin(s);
   hipLaunchKernelGGL(c.fn, dim3((unsigned)G), dim3(c.threads), c.lds, s, a);
+  phpc_ktime_end(kt, s);
   PHPC_HIP(hipGetLastError());
   return PHPC_OK;
 }
@@ -1131,9 +1136,11 @@ int phpc_dgemm_split_ws(const double *dA, long lda, const double *dB, long ldb, 
     const int logk = (int)std::ceil(std::log2((double)std::max(k, 2)));
     int pbits = 0, L = 0;
     if (slices > 0) {
-      // the caller fixes the modulus count; use the widest integers it can hold
+      // the caller fixes the modulus count; use the widest integers it can hold, but
+      // no wider than residues_of takes: |A'| <= 2^p (|a| < 2^e), and p <= kMaxIntBits
+      // keeps every integer below 2^72, its nine-byte range
       L = std::min(slices, kMaxMods);
-      pbits = (int)std::floor((crt_bits(L) - logk - 2) / 2);
+      pbits = std::min(kMaxIntBits, (int)std::floor((crt_bits(L) - logk - 2) / 2));
     } else {
       const double target = target_err > 0 ? target_err : 1e-14;
       const double R = std::max(min_positive(ra), min_positive(rb));

@@ -30,6 +30,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include "fill.h"
 #include "phpc_summa.h"
 #include "utils.h"
 
@@ -38,32 +39,6 @@
     fprintf(stderr, "Check at " __FILE__ " line %d failed\n", __LINE__); \
     exit(EXIT_FAILURE);                                                  \
   }
-
-/* element i of the seeded streams, as phpc_fill_device generates them */
-static uint64_t mix64(uint64_t z) {
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-  return z ^ (z >> 31);
-}
-
-static void fill(double *x, size_t n, const char *dist, uint64_t seed) {
-  for (size_t i = 0; i < n; ++i) {
-    if (strcmp(dist, "ramp") == 0) {
-      x[i] = (double)i;
-      continue;
-    }
-    uint64_t z = mix64(seed + (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ULL);
-    if (strcmp(dist, "uniform") == 0) {
-      x[i] = 2.0 * ((double)(z >> 11) * 0x1.0p-53) - 1.0;
-    } else {
-      int e = (int)(((z >> 32) * 41ULL) >> 32) - 20;
-      double v = 1.0;
-      for (; e > 0; --e) v *= 2.0;
-      for (; e < 0; ++e) v *= 0.5;
-      x[i] = (z & 1) ? -v : v;
-    }
-  }
-}
 
 int main(int argc, char *argv[]) {
   int pos = 0;
@@ -177,8 +152,8 @@ int main(int argc, char *argv[]) {
     return EXIT_FAILURE;
   }
 
-  fill(A, nn, dist, seed);
-  fill(B, nn, dist, seed + 1);
+  phpc_host_fill(A, nn, dist, seed); /* dist was validated above */
+  phpc_host_fill(B, nn, dist, seed + 1);
 
   /* CUDA (here: HIP MFMA) pass */
   memset(C, 0, nn * sizeof(double));

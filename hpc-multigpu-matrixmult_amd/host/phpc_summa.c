@@ -122,12 +122,18 @@ static int blk_start(int n, int parts, int i) {
 int phpc_gemm_summa(gemm_t f, phpc_grid_t grid, const double *A, const double *B, double *C, int N, int gpu_count,
                     int grid_width, int grid_height, int block_width, float *gpu_time) {
   if (gpu_time) *gpu_time = 0.0f;
-  if (!f || !grid || N < 0 || (N > 0 && (!A || !B || !C))) return PHPC_ERR_ARG;
-  if (grid->p > N || grid->q > N) return PHPC_ERR_ARG;
+  if (!f || !grid || N < 0 || (N > 0 && (!A || !B || !C)))
+    return phpc_set_error(PHPC_ERR_ARG, "bad SUMMA loop arguments");
+  if (grid->p > N || grid->q > N)
+    return phpc_set_error(PHPC_ERR_ARG, "grid %dx%d larger than N = %d", grid->p, grid->q, N);
+  /* A status left by an earlier call on this thread must not stop the loop: only an
+   * error raised by f during this call does. (A user-supplied gemm_t that never
+   * records a status then always reads back PHPC_OK.) */
+  phpc_set_error(PHPC_OK, NULL);
   if (N == 0) return PHPC_OK;
   const int lcm = find_lcm(grid->p, grid->q);
   const int w = grid->panel > 0 ? grid->panel : N / lcm;
-  if (w < 1) return PHPC_ERR_ARG;
+  if (w < 1) return phpc_set_error(PHPC_ERR_ARG, "panel width %d", w);
   float total = 0.0f;
   for (int k0 = 0; k0 < N; k0 += w) {
     const int kw = k0 + w <= N ? w : N - k0;

@@ -80,6 +80,8 @@ _SIGS = {
     "phpc_last_error": (I, []),
     "phpc_last_error_message": (ctypes.c_char_p, []),
     "phpc_init": (I, []),
+    "phpc_kernel_timing": (I, [I]),
+    "phpc_kernel_timing_read": (I, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(I)]),
     "phpc_dgemm_device": (I, [vp, L, vp, L, vp, L, I, I, I, I, vp]),
     "phpc_dgemm_vendor_device": (I, [vp, L, vp, L, vp, L, I, I, I, vp]),
     "phpc_set_launch_policy": (I, [I, I, I]),
@@ -317,6 +319,19 @@ def check_sampled(A, B, C, rows, cols, C0=None, stream=None):
                                    rows.ctypes.data_as(c_int_p), cols.ctypes.data_as(c_int_p),
                                    len(rows), ctypes.byref(cw), ctypes.byref(nw), stream_ptr(stream)))
     return cw.value, nw.value
+
+
+def kernel_timing(on=True):
+    """Start (or stop) event timing of the main GEMM kernel launches; clears earlier records."""
+    check(lib().phpc_kernel_timing(1 if on else 0))
+
+
+def kernel_timing_read():
+    """(summed milliseconds, launch count) of the GEMM kernel launches timed since the last
+    read; waits for them to finish."""
+    ms, n = ctypes.c_double(), I()
+    check(lib().phpc_kernel_timing_read(ctypes.byref(ms), ctypes.byref(n)))
+    return ms.value, n.value
 
 
 def set_launch_policy(honor_grid=0, config_id=-1, grid=0):

@@ -56,7 +56,8 @@ typedef struct phpc_server *phpc_server_t;
  * workspaces are sized up front for requests up to max_dim x max_dim x max_dim;
  * batch_max: most FP32 requests folded into one grouped launch. */
 int phpc_server_create(size_t budget_bytes, int slots, int max_dim, int batch_max, phpc_server_t *server);
-/* Enqueue (non-blocking). PHPC_ERR_BUDGET if the request can never fit the budget. */
+/* Enqueue (non-blocking). PHPC_ERR_BUDGET if the request can never fit the budget, or if
+ * any of m, n, k exceeds max_dim (its workspaces would have to grow outside the budget). */
 int phpc_server_submit(phpc_server_t server, const phpc_request *req, uint64_t *ticket);
 /* Block until the request is done; the ticket is released afterwards. */
 int phpc_server_wait(phpc_server_t server, uint64_t ticket, phpc_result *result);

@@ -53,6 +53,8 @@ def lib():
         h.orc_block_start.restype = I
         h.orc_block_start.argtypes = [I, I, I]
         h.orc_summa_emul.argtypes = [_dp, _dp, _dp, I, I, I, I]
+        L_ = ctypes.c_long
+        h.orc_summa_entries.argtypes = [_dp, L_, _dp, L_, _dp, L_, _ip, _ip, L_, I, I, _dp]
         h.orc_gemm_dd.argtypes = [_dp, I, _dp, I, _dp, I, _dp, I, I, I, I]
         h.orc_gemm_dd_entries.argtypes = [_dp, I, _dp, I, _dp, I, _ip, _ip, _dp, ctypes.c_size_t, I]
         h.orc_dot_q.restype = ctypes.c_double
@@ -122,6 +124,24 @@ def kernel_emul(A, B, C, m=None, k=None, n=None, lda=None, ldb=None, ldc=None):
 def summa_emul(A, B, C, p, q, panel=0):
     lib().orc_summa_emul(_p(A), _p(B), _p(C), A.shape[0], p, q, panel)
     return C
+
+
+def summa_entries(A, B, rows, cols, panel=0, C0=None):
+    """Entries (rows[e], cols[e]) of the reference arithmetic for C0 + A B: per-panel FMA
+    chains from zero added to C in panel order (panel <= 0: one chain over all of k)."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    r = np.ascontiguousarray(rows, dtype=np.int32)
+    c = np.ascontiguousarray(cols, dtype=np.int32)
+    out = np.empty(len(r))
+    c0 = None
+    ldc0 = 0
+    if C0 is not None:
+        C0 = np.ascontiguousarray(C0, dtype=np.float64)
+        c0, ldc0 = _p(C0), C0.shape[1]
+    lib().orc_summa_entries(_p(A), A.shape[1], _p(B), B.shape[1], c0, ldc0,
+                            r.ctypes.data_as(_ip), c.ctypes.data_as(_ip), len(r), A.shape[1], panel, _p(out))
+    return out
 
 
 def find_lcm(a, b):

@@ -124,6 +124,33 @@ void orc_summa_emul(const double *A, const double *B, double *C, int N, int p, i
   }
 }
 
+/*
+ * The reference's SUMMA arithmetic (src/phpc_summa.c:63-95 over src/phpc_gemm.cu:32,50-55)
+ * for a list of entries of C = C0 + A B, k x k panels of width `panel` (<= 0: one panel,
+ * which is one local-GEMM call over the whole k): out[e] = ((c0 + s_0) + s_1) + ... with
+ * s_t = fma chain from zero over panel t in ascending k. Every rank of a SUMMA grid walks
+ * the same global k panels, so the value of an entry does not depend on the grid. This is
+ * orc_summa_emul / orc_kernel_emul evaluated only where asked, for sizes where the full
+ * emulation would take too long; C0 may be NULL (zero).
+ */
+void orc_summa_entries(const double *A, long lda, const double *B, long ldb, const double *C0, long ldc0,
+                       const int *rows, const int *cols, long count, int k, int panel, double *out) {
+  const int w = panel > 0 ? panel : (k > 0 ? k : 1);
+#pragma omp parallel for schedule(static)
+  for (long e = 0; e < count; ++e) {
+    const double *arow = A + (size_t)rows[e] * lda;
+    const double *bcol = B + cols[e];
+    double c = C0 ? C0[(size_t)rows[e] * ldc0 + cols[e]] : 0.0;
+    for (int k0 = 0; k0 < k; k0 += w) {
+      const int k1 = k0 + w < k ? k0 + w : k;
+      double acc = 0.0;
+      for (int kk = k0; kk < k1; ++kk) acc = fma(arow[kk], bcol[(size_t)kk * ldb], acc);
+      c = c + acc;
+    }
+    out[e] = c;
+  }
+}
+
 /* ---------------- double-double reference ---------------- */
 
 static inline void two_sum(double a, double b, double *s, double *e) {

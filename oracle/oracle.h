@@ -74,6 +74,13 @@ int orc_block_start(int n, int parts, int i);
  * Rank blocks follow orc_block_size. Pinned by: equality with orc_kernel_emul for one
  * panel, the ramp closed form, and a hand-computed grouping known answer. */
 void orc_summa_emul(const double *A, const double *B, double *C, int N, int p, int q, int panel);
+/* orc_summa_entries: the value orc_summa_emul (panel > 0) or orc_kernel_emul (panel <= 0)
+ * gives entry (rows[e], cols[e]) of C0 + A B with k = N, computed for the listed entries
+ * only: out[e] = ((c0 + s_0) + s_1) + ..., s_t the FMA chain from zero over panel t. C0
+ * may be NULL. Pinned by: equality with orc_summa_emul and orc_kernel_emul over every
+ * entry of small matrices, for several grids and panel widths (tests/test_oracle.py). */
+void orc_summa_entries(const double *A, long lda, const double *B, long ldb, const double *C0, long ldc0,
+                       const int *rows, const int *cols, long count, int k, int panel, double *out);
 
 /* ---------------- independent reference values ----------------
  * orc_gemm_dd: C_ref[i,j] = fl(C0[i,j] + sum_k A[i,k] B[k,j]) computed in

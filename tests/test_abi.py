@@ -93,3 +93,49 @@ def test_host_summa_rejects_grid_larger_than_n(lib):
     t = ctypes.c_float(0)
     assert lib.phpc_gemm_summa(cb, g.h, phpc._ptr(A), phpc._ptr(A), phpc._ptr(A), 2, 1, 1, 1, 1,
                                ctypes.byref(t)) == phpc.PHPC_ERR_ARG
+
+
+def test_reference_header_name_compiles_and_links(tmp_path, lib):
+    """A C file written against the reference's headers (it includes "phpc_gemm.cuh", as
+    src/phpc_summa.c:5 does, and "phpc_summa.h") compiles with gcc against include/ and links
+    against libphpc.so; the program only takes the entry points' addresses (no GPU call)."""
+    src = tmp_path / "caller.c"
+    src.write_text(
+        '#include "phpc_gemm.cuh"\n'
+        '#include "phpc_summa.h"\n'
+        '#include <stdio.h>\n'
+        'typedef void (*summa_fn)(phpc_grid_t, const double *, const double *, double *, int, int, int, int, int,\n'
+        '                         float *);\n'
+        'int main(void) {\n'
+        '  gemm_t f = phpc_gemm_cuda, g = phpc_gemm_cublas;\n'
+        '  summa_fn s = phpc_gemm_summa_cuda;\n'
+        '  printf("%d\\n", f != 0 && g != 0 && s != 0 && find_lcm(4, 6) == 12);\n'
+        '  return 0;\n'
+        '}\n')
+    exe = tmp_path / "caller"
+    libdir = phpc.LIB_PATH.parent
+    subprocess.run(["gcc", "-Wall", "-Werror", "-I", str(ROOT / "include"), str(src), "-o", str(exe),
+                    "-L", str(libdir), "-lphpc", f"-Wl,-rpath,{libdir}", "-L/opt/rocm/lib",
+                    "-Wl,-rpath,/opt/rocm/lib", "-lamdhip64"], check=True, capture_output=True, text=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True, timeout=60).stdout
+    assert out.strip() == "1"
+
+
+def test_host_summa_loop_ignores_a_stale_error(lib):
+    """An error left on this thread by an earlier call must not stop the generic loop after its
+    first block: a gemm_t that records no status (this Python callback) runs every step."""
+    A = np.zeros((4, 4))
+    t = ctypes.c_float(0)
+    # block_width 0 is invalid: the call records PHPC_ERR_ARG (or NODEV without a GPU)
+    lib.phpc_gemm_cuda(phpc._ptr(A), 4, phpc._ptr(A), 4, phpc._ptr(A), 4, 4, 4, 4, 1, 1, 1, 0, ctypes.byref(t))
+    assert lib.phpc_last_error() != 0
+    N, p, q = 12, 2, 2
+    A, B, C0 = O.uniform((N, N), 1), O.uniform((N, N), 2), O.uniform((N, N), 3)
+    cb, calls = _oracle_gemm_t()
+    g = phpc.Grid(dims=(p, q))
+    C = C0.copy()
+    rc = lib.phpc_gemm_summa(cb, g.h, phpc._ptr(A), phpc._ptr(B), phpc._ptr(C), N, 1, 8, 8, 32, ctypes.byref(t))
+    assert rc == 0 and lib.phpc_last_error() == 0
+    assert len(calls) == (N // (N // O.find_lcm(p, q))) * p * q
+    want = O.summa_emul(A, B, C0.copy(), p, q, 0)
+    assert np.array_equal(C.view(np.uint64), want.view(np.uint64))

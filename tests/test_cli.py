@@ -44,3 +44,38 @@ def test_bad_mode(tmp_path):
     r = run(["16", "32", "1", "1", "t", "--mode", "bogus"], tmp_path)
     assert r.returncode != 0
     assert "--mode must be" in r.stderr
+
+
+ITER = phpc.BIN_PATH.parent / "iterative.out"
+
+
+@pytest.mark.parametrize("dist,n", [("ramp", 1), ("ramp", 40), ("uniform", 33), ("wide", 48)])
+def test_iterative_out_matches_oracle(tmp_path, dist, n):
+    """bin/iterative.out, the harness's CPU path (src/iterative.c:15-46): prints "n,seconds"
+    and its C is bit-identical to the oracle's restatement of the reference loop on the
+    same generated inputs (which the device fill also produces)."""
+    import numpy as np
+    import oracle as O
+    out = tmp_path / "c.bin"
+    args = [str(ITER), str(n), "--out", str(out)]
+    if dist != "ramp":
+        args += ["--dist", dist, "--seed", "5"]
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    size, secs = r.stdout.strip().split(",")
+    assert int(size) == n and float(secs) >= 0.0
+    gen = {"ramp": lambda s, seed: O.ramp(s), "uniform": O.uniform, "wide": O.wide}[dist]
+    A, B = gen((n, n), 5), gen((n, n), 6)
+    want = O.iterative(A, B, np.zeros((n, n)))
+    got = np.fromfile(out, dtype=np.float64).reshape(n, n)
+    assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
+    if dist == "ramp":  # the closed form of the reference's own input
+        assert np.array_equal(got, O.ramp_closed_matrix(n))
+
+
+def test_iterative_out_usage_and_errors(tmp_path):
+    r = subprocess.run([str(ITER)], capture_output=True, text=True, timeout=30)
+    assert r.returncode != 0 and "Usage:" in r.stderr
+    for bad in (["8", "--dist", "normal"], ["8", "--bogus"], ["0"]):
+        r = subprocess.run([str(ITER)] + bad, capture_output=True, text=True, timeout=30)
+        assert r.returncode != 0 and "Error" in r.stderr, bad

@@ -285,3 +285,23 @@ def test_panel_grouped_configs_on_ragged_panels(cfg, m, k, n, panel):
     finally:
         phpc.set_launch_policy(0, -1, 0)
     assert np.array_equal(bits(dC.cpu().numpy()), bits(want))
+
+
+def test_kernel_timing_counts_gemm_launches():
+    """phpc_kernel_timing brackets each FP64 tile launch with events on its stream: one
+    device GEMM is one timed launch of positive duration; nothing is recorded while off."""
+    import torch
+    N = 1024
+    dA = torch.empty((N, N), dtype=torch.float64, device="cuda")
+    dB = torch.empty_like(dA)
+    phpc.fill_device(dA, phpc.DIST_UNIFORM, 1)
+    phpc.fill_device(dB, phpc.DIST_UNIFORM, 2)
+    dC = torch.zeros_like(dA)
+    phpc.kernel_timing(True)
+    phpc.dgemm_device(dA, dB, dC)
+    ms, n = phpc.kernel_timing_read()
+    assert n == 1 and ms > 0.0
+    phpc.kernel_timing(False)
+    phpc.dgemm_device(dA, dB, dC)
+    torch.cuda.synchronize()
+    assert phpc.kernel_timing_read() == (0.0, 0)

@@ -74,7 +74,7 @@ def out_array(acc, m, n):
 
 @pytest.mark.parametrize("policy", [phpc.SCHED_FIFO, phpc.SCHED_SJF, phpc.SCHED_LANES])
 def test_alone_vs_mix_vs_one_shot(policy):
-    srv = phpc.Server(8 * GiB, slots=3, max_dim=1024, batch_max=8)
+    srv = phpc.Server(8 * GiB, slots=3, max_dim=2304, batch_max=8)
     srv.set_split_min(SPLIT_MIN)
     srv.set_policy(policy, 0.0)
     try:
@@ -123,18 +123,45 @@ def test_alone_vs_mix_vs_one_shot(policy):
 
 
 def test_budget_admission():
-    srv = phpc.Server(3 * GiB, slots=2, max_dim=256, batch_max=4)
+    import torch
+    torch.cuda.synchronize()
+    free0, _ = torch.cuda.mem_get_info()
+    # max_dim 4096: the two slots' workspaces take about 1.9 GiB of the 4 GiB, leaving an
+    # operand arena of about 2.1 GiB
+    srv = phpc.Server(4 * GiB, slots=2, max_dim=4096, batch_max=4)
     try:
         # a request whose operands alone exceed the budget is refused at submit
         with pytest.raises(RuntimeError, match="error 6"):
             srv.submit(phpc.ACC_FP64, 16384, 16384, 16384)
-        # requests of ~0.8 GiB each: at most three fit at once
-        big = (phpc.ACC_FP32, 8192, 8192, 8192)
-        ts = [srv.submit(*big, 0, 50 + i) for i in range(4)] + [srv.submit(phpc.ACC_FP64, 200, 200, 200, 0, 60)]
+        # FP64 requests of 384 MiB each (split path) and FP32 ones of 192 MiB (3 x BF16
+        # slice planes): at most five of the former fit at once, so admission must wait
+        ts = [srv.submit(phpc.ACC_FP64, 4096, 4096, 4096, 0, 50 + i) for i in range(7)]
+        ts += [srv.submit(phpc.ACC_FP32, 4096, 4096, 4096, 0, 70 + i) for i in range(3)]
+        ts += [srv.submit(phpc.ACC_FP64, 200, 200, 200, 0, 60)]
         for t in ts:
             assert srv.wait(t)["status"] == 0
         in_use, peak = srv.memory()
-        assert in_use == 0 and 0 < peak <= 3 * GiB
+        assert in_use == 0 and 0 < peak <= 4 * GiB
+        # the device memory the server holds (workspaces + arena) stays within the budget:
+        # nothing grows on the request path
+        torch.cuda.synchronize()
+        free1, _ = torch.cuda.mem_get_info()
+        assert free0 - free1 <= 4 * GiB + 64 * 2 ** 20, (free0 - free1) / GiB
+    finally:
+        srv.close()
+
+
+def test_requests_wider_than_max_dim_are_refused():
+    """The per-slot workspaces are sized for max_dim at creation; a wider request is refused at
+    submit instead of growing them outside the budget, and the server keeps serving."""
+    srv = phpc.Server(1 * GiB, slots=1, max_dim=256, batch_max=4)
+    try:
+        for shape in ((257, 8, 8), (8, 257, 8), (8, 8, 257)):
+            with pytest.raises(RuntimeError, match="error 6"):
+                srv.submit(phpc.ACC_FP32, *shape)
+        assert "max_dim" in phpc.lib().phpc_last_error_message().decode()
+        t = srv.submit(phpc.ACC_FP64, 256, 256, 256, 0, 5)
+        assert srv.wait(t)["status"] == 0
     finally:
         srv.close()
 

@@ -127,7 +127,8 @@ def test_split_large_sampled_accuracy(dist, method):
 
 @pytest.mark.parametrize("m,k,n,L,gen", [(1, 1, 1, 3, "u"), (7, 9, 5, 6, "u"), (64, 64, 64, 14, "u"),
                                          (200, 300, 130, 14, "w"), (257, 129, 255, 10, "u"),
-                                         (96, 1000, 80, 16, "w"), (33, 65, 300, 4, "u")])
+                                         (96, 1000, 80, 16, "w"), (33, 65, 300, 4, "u"),
+                                         (16, 64, 16, 20, "u"), (8, 3, 8, 20, "w"), (40, 17, 24, 18, "u")])
 def test_crt_matches_restatement_bitwise(m, k, n, L, gen):
     import torch
     print(f"crt m={m} k={k} n={n} L={L} {gen}", flush=True)
@@ -137,6 +138,9 @@ def test_crt_matches_restatement_bitwise(m, k, n, L, gen):
     info = phpc.dgemm_split_device(dev(A), dev(B), dC, slices=L, method=phpc.SPLIT_CRT)
     torch.cuda.synchronize()
     assert info.method == 1 and info.slices == L
+    # with many moduli and a short k the widest integers that fit would pass 2^72, the
+    # residue routine's range; the width is capped at 71 bits there
+    assert 1 <= info.int_bits <= 71
     want = O.crt_gemm(A, B, C0.copy(), info.int_bits, L)
     assert np.array_equal(bits(dC.cpu().numpy()), bits(want))
 

@@ -347,3 +347,32 @@ def test_crt_gemm_within_integerisation_bound():
                 assert cw < prev
             prev = cw
         assert prev <= 1e-14
+
+
+@pytest.mark.parametrize("N,p,q,panel", [(24, 1, 1, 0), (30, 2, 3, 7), (33, 3, 3, 0), (40, 4, 4, 5), (17, 2, 2, 17)])
+def test_summa_entries_equals_full_emulation(N, p, q, panel):
+    """orc_summa_entries, the per-entry form used at sizes the full emulation cannot reach,
+    gives every entry of orc_summa_emul (any grid, this panel) and, for one panel, of
+    orc_kernel_emul, bit for bit, with and without C0; on wide-exponent inputs, where a
+    different panel grouping changes the rounding."""
+    A, B, C0 = O.wide((N, N), 1), O.wide((N, N), 2), O.uniform((N, N), 3)
+    rr, cc = np.meshgrid(np.arange(N), np.arange(N), indexing="ij")
+    rows, cols = rr.ravel(), cc.ravel()
+    w = panel if panel else N // O.find_lcm(p, q)
+    want = O.summa_emul(A, B, C0.copy(), p, q, panel)
+    got = O.summa_entries(A, B, rows, cols, panel=w, C0=C0).reshape(N, N)
+    assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
+    one = O.summa_entries(A, B, rows, cols, panel=0).reshape(N, N)
+    assert np.array_equal(one.view(np.uint64), O.kernel_emul(A, B, np.zeros((N, N))).view(np.uint64))
+    # the grouping matters on these inputs: a mistaken single chain would not pass
+    if 0 < w < N:
+        assert not np.array_equal(one.view(np.uint64), O.summa_emul(A, B, np.zeros((N, N)), p, q, panel).view(np.uint64))
+
+
+def test_summa_entries_ramp_closed_form():
+    N = 64
+    A = B = O.ramp((N, N))
+    rng = np.random.default_rng(0)
+    rows, cols = rng.integers(0, N, 300), rng.integers(0, N, 300)
+    got = O.summa_entries(A, B, rows, cols, panel=16)
+    assert np.array_equal(got, O.ramp_closed_matrix(N)[rows, cols])
