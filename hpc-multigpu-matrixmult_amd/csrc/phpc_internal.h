@@ -221,6 +221,12 @@ struct Workspace {
   static constexpr int kSlots = 8;
   void *p[kSlots] = {};
   size_t cap[kSlots] = {};
+  // Fixed region (set by use_fixed): get() allocates nothing and instead carves each
+  // slot, in call order, out of [fixed, fixed + fixed_size) at 256-byte aligned offsets,
+  // returning nullptr once the region is exhausted; release() frees nothing.
+  char *fixed = nullptr;
+  size_t fixed_size = 0, fixed_used = 0;
+  void use_fixed(void *base, size_t size);
   void *get(size_t bytes, int slot, hipStream_t stream);
   size_t total() const;
   void release();

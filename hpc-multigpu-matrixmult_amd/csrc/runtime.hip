@@ -191,9 +191,25 @@ void *phpc_scratch(size_t bytes, int slot) {
   return s.p;
 }
 
+void Workspace::use_fixed(void *base, size_t size) {
+  release();
+  fixed = static_cast<char *>(base);
+  fixed_size = size;
+  fixed_used = 0;
+}
+
 void *Workspace::get(size_t bytes, int slot, hipStream_t stream) {
   if (slot < 0 || slot >= kSlots) return nullptr;
   if (bytes == 0) bytes = 1;
+  if (fixed) {
+    if (bytes <= cap[slot]) return p[slot];
+    const size_t off = (fixed_used + 255) / 256 * 256;
+    if (off > fixed_size || bytes > fixed_size - off) return nullptr;
+    p[slot] = fixed + off;
+    cap[slot] = bytes;
+    fixed_used = off + bytes;
+    return p[slot];
+  }
   if (bytes > cap[slot]) {
     if (p[slot]) {
       (void)hipStreamSynchronize(stream);
@@ -217,12 +233,12 @@ size_t Workspace::total() const {
 }
 
 void Workspace::release() {
-  for (int i = 0; i < kSlots; ++i)
-    if (p[i]) {
-      (void)hipFree(p[i]);
-      p[i] = nullptr;
-      cap[i] = 0;
-    }
+  for (int i = 0; i < kSlots; ++i) {
+    if (p[i] && !fixed) (void)hipFree(p[i]);
+    p[i] = nullptr;
+    cap[i] = 0;
+  }
+  fixed_used = 0;
 }
 
 extern "C" int phpc_set_launch_policy(int honor_grid, int config_id, int grid) {

@@ -134,6 +134,53 @@ __global__ void zero_words_kernel(uint32_t *p, long nwords) {
   for (long w = (long)blockIdx.x * blockDim.x + threadIdx.x; w < nwords; w += (long)gridDim.x * blockDim.x) p[w] = 0u;
 }
 
+// What the modulus-count rule needs from the row and column ratios (all >= 0 for finite
+// inputs): their maxima, their smallest positive values (+inf when none is positive),
+// and the non-finite flag. max and min are exact, so the result does not depend on the
+// reduction order.
+struct RatioSummary {
+  double max_a, max_b, minpos_a, minpos_b;
+  int bad;
+  int pad;
+};
+
+__global__ __launch_bounds__(256) void ratio_summary_kernel(const double *__restrict__ ra, int m,
+                                                            const double *__restrict__ rb, int n,
+                                                            const int *__restrict__ bad,
+                                                            RatioSummary *__restrict__ out) {
+  __shared__ double red[4][256];
+  double v[4] = {0.0, 0.0, INFINITY, INFINITY};
+  for (int i = threadIdx.x; i < m; i += 256) {
+    const double x = ra[i];
+    if (x > v[0]) v[0] = x;
+    if (x > 0.0 && x < v[2]) v[2] = x;
+  }
+  for (int j = threadIdx.x; j < n; j += 256) {
+    const double x = rb[j];
+    if (x > v[1]) v[1] = x;
+    if (x > 0.0 && x < v[3]) v[3] = x;
+  }
+  for (int q = 0; q < 4; ++q) red[q][threadIdx.x] = v[q];
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + h]);
+      red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + h]);
+      red[2][threadIdx.x] = fmin(red[2][threadIdx.x], red[2][threadIdx.x + h]);
+      red[3][threadIdx.x] = fmin(red[3][threadIdx.x], red[3][threadIdx.x + h]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out->max_a = red[0][0];
+    out->max_b = red[1][0];
+    out->minpos_a = red[2][0];
+    out->minpos_b = red[3][0];
+    out->bad = *bad;
+    out->pad = 0;
+  }
+}
+
 // Column maxima / sums of B over a chunk of rows; combined with atomics.
 __global__ __launch_bounds__(256) void col_stats_kernel(const double *__restrict__ B, long ldb, int k, int n,
                                                         int rows_per_block, unsigned long long *__restrict__ cmax,
@@ -518,6 +565,20 @@ __global__ __launch_bounds__(512) void i8_level_256x256_w8_sd8(LevelArgs a) { i8
 __global__ __launch_bounds__(1024) void i8_level_256x256_w16_k128(LevelArgs a) {
   i8_level_body<256, 256, 4, 4, 128, 2>(a);
 }
+// deeper rings: a 256x128 tile with three 128-byte stages (48 KiB each), and the
+// 16-wave 256x256 tile with four or five 64-byte stages (32 KiB each)
+__global__ __launch_bounds__(512) void i8_level_256x128_w8_k128_s3(LevelArgs a) {
+  i8_level_body<256, 128, 4, 2, 128, 3>(a);
+}
+__global__ __launch_bounds__(1024) void i8_level_256x256_w16_k64_s4(LevelArgs a) {
+  i8_level_body<256, 256, 4, 4, 64, 4>(a);
+}
+__global__ __launch_bounds__(1024) void i8_level_256x256_w16_k64_s5(LevelArgs a) {
+  i8_level_body<256, 256, 4, 4, 64, 5>(a);
+}
+__global__ __launch_bounds__(1024) void i8_level_256x128_w16_k128_s3(LevelArgs a) {
+  i8_level_body<256, 128, 8, 2, 128, 3>(a);
+}
 // BF16 slices on the same pipeline (FP32-accurate 3 x BF16 GEMM)
 __global__ __launch_bounds__(512) void bf16_level_256x256_w8_k128(LevelArgs a) {
   i8_level_body<256, 256, 2, 4, 128, 2, 16, true>(a);
@@ -547,6 +608,10 @@ const I8Config kI8Configs[] = {
     I8CFG(256, 256, 2, 4, 128, 2, i8_level_256x256_w8_k128_sd8),  // 6: as 2, 32-bank swizzle
     I8CFG(256, 256, 2, 4, 64, 3, i8_level_256x256_w8_sd8),        // 7: as 0, 32-bank swizzle
     I8CFG(256, 256, 4, 4, 128, 2, i8_level_256x256_w16_k128),     // 8: 16 waves, 64x64 wave tiles (default)
+    I8CFG(256, 128, 4, 2, 128, 3, i8_level_256x128_w8_k128_s3),   // 9: 8 waves, 3 stages of 128-byte slabs
+    I8CFG(256, 256, 4, 4, 64, 4, i8_level_256x256_w16_k64_s4),    // 10: 16 waves, 4 stages of 64-byte slabs
+    I8CFG(256, 256, 4, 4, 64, 5, i8_level_256x256_w16_k64_s5),    // 11: 16 waves, 5 stages of 64-byte slabs
+    I8CFG(256, 128, 8, 2, 128, 3, i8_level_256x128_w16_k128_s3),  // 12: 16 waves (32x64 wave tiles), 3 stages
 };
 constexpr int kNumI8 = sizeof(kI8Configs) / sizeof(kI8Configs[0]);
 int g_i8_big = 8;  // kI8Configs index for large problems
@@ -1002,6 +1067,13 @@ CrtConsts make_crt(int L) {
   return c;
 }
 
+// Bytes of the statistics block (slot 3): exponents (m + n ints), ratios (m + n
+// doubles, 16-byte aligned), column maxima and sums (2 n words), the non-finite flag and
+// the ratio summary.
+size_t split_meta_bytes(int m, int n) {
+  return (size_t)(m + n) * (sizeof(int) + sizeof(double)) + (size_t)n * 16 + 64 + sizeof(RatioSummary) + 16;
+}
+
 double crt_bits(int L) {
   double b = 0;
   for (int t = 0; t < L; ++t) b += std::log2((double)kModuli[t]);
@@ -1021,13 +1093,6 @@ double crt_needed_bits(int p, double R, int k) {
   return p + std::log2((double)std::max(k, 1)) + std::log2(std::ldexp(1.0, p) / r + 0.5) + 2.0;
 }
 
-// smallest positive entry (1 when there is none)
-double min_positive(const std::vector<double> &v) {
-  double m = 0.0;
-  for (double x : v)
-    if (x > 0.0 && (m == 0.0 || x < m)) m = x;
-  return m > 0.0 ? m : 1.0;
-}
 
 // a-priori estimate for integer operands of p bits: each entry moves by at most
 // 2^(e - p - 1); the errors add like a random walk over k. The factor 2 covers the tail
@@ -1096,7 +1161,7 @@ int phpc_dgemm_split_ws(const double *dA, long lda, const double *dB, long ldb, 
   const long kp = ((long)k + 63) / 64 * 64;
 
   // scratch: exponents, ratios, column stats (slices and the FP64 accumulator below)
-  const size_t small = (size_t)(m + n) * (sizeof(int) + sizeof(double)) + (size_t)n * 16 + 64;
+  const size_t small = split_meta_bytes(m, n);
   char *meta = (char *)ws->get(small, 3, s);
   if (!meta) return phpc_set_error(PHPC_ERR_MEM, "split scratch allocation failed");
   int *eA = (int *)meta;
@@ -1106,6 +1171,7 @@ int phpc_dgemm_split_ws(const double *dA, long lda, const double *dB, long ldb, 
   unsigned long long *cmax = (unsigned long long *)(ratioB + n);
   double *csum = (double *)(cmax + n);
   int *bad = (int *)(csum + n);
+  RatioSummary *summary = (RatioSummary *)(((uintptr_t)(bad + 1) + 15) & ~(uintptr_t)15);
 
   {
     const long words = ((long)n * 16 + 16) / 4;  // cmax, csum and the flag
@@ -1124,15 +1190,13 @@ int phpc_dgemm_split_ws(const double *dA, long lda, const double *dB, long ldb, 
 
   if (use_crt) {
     // CRT scheme: choose the integer width p, then the modulus count
-    std::vector<double> ra(m), rb(n);
-    int hbad = 0;
-    PHPC_HIP(hipMemcpyAsync(ra.data(), ratioA, sizeof(double) * m, hipMemcpyDeviceToHost, s));
-    PHPC_HIP(hipMemcpyAsync(rb.data(), ratioB, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-    PHPC_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+    RatioSummary hs{};
+    hipLaunchKernelGGL(ratio_summary_kernel, dim3(1), dim3(256), 0, s, ratioA, m, ratioB, n, bad, summary);
+    PHPC_HIP(hipGetLastError());
+    PHPC_HIP(hipMemcpyAsync(&hs, summary, sizeof(hs), hipMemcpyDeviceToHost, s));
     PHPC_HIP(hipStreamSynchronize(s));
-    if (hbad) return phpc_set_error(PHPC_ERR_ARG, "split-precision GEMM needs finite inputs");
-    const double mra = *std::max_element(ra.begin(), ra.end());
-    const double mrb = *std::max_element(rb.begin(), rb.end());
+    if (hs.bad) return phpc_set_error(PHPC_ERR_ARG, "split-precision GEMM needs finite inputs");
+    const double mra = hs.max_a, mrb = hs.max_b;
     const int logk = (int)std::ceil(std::log2((double)std::max(k, 2)));
     int pbits = 0, L = 0;
     if (slices > 0) {
@@ -1143,7 +1207,9 @@ int phpc_dgemm_split_ws(const double *dA, long lda, const double *dB, long ldb, 
       pbits = std::min(kMaxIntBits, (int)std::floor((crt_bits(L) - logk - 2) / 2));
     } else {
       const double target = target_err > 0 ? target_err : 1e-14;
-      const double R = std::max(min_positive(ra), min_positive(rb));
+      // the smallest positive ratio of each side, 1 when there is none
+      const double R = std::max(std::isinf(hs.minpos_a) ? 1.0 : hs.minpos_a,
+                                std::isinf(hs.minpos_b) ? 1.0 : hs.minpos_b);
       for (pbits = 8; pbits < 70; ++pbits)
         if (crt_error_estimate(pbits, mra, mrb, k) <= target) break;
       for (L = 2; L < kMaxMods; ++L)
@@ -1206,15 +1272,13 @@ int phpc_dgemm_split_ws(const double *dA, long lda, const double *dB, long ldb, 
   double est = 0.0;
   {
     // exponents and ratios decide the slice count (and reveal non-finite input)
-    std::vector<double> ra(m), rb(n);
-    int hbad = 0;
-    PHPC_HIP(hipMemcpyAsync(ra.data(), ratioA, sizeof(double) * m, hipMemcpyDeviceToHost, s));
-    PHPC_HIP(hipMemcpyAsync(rb.data(), ratioB, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-    PHPC_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+    RatioSummary hs{};
+    hipLaunchKernelGGL(ratio_summary_kernel, dim3(1), dim3(256), 0, s, ratioA, m, ratioB, n, bad, summary);
+    PHPC_HIP(hipGetLastError());
+    PHPC_HIP(hipMemcpyAsync(&hs, summary, sizeof(hs), hipMemcpyDeviceToHost, s));
     PHPC_HIP(hipStreamSynchronize(s));
-    if (hbad) return phpc_set_error(PHPC_ERR_ARG, "split-precision GEMM needs finite inputs");
-    const double mra = *std::max_element(ra.begin(), ra.end());
-    const double mrb = *std::max_element(rb.begin(), rb.end());
+    if (hs.bad) return phpc_set_error(PHPC_ERR_ARG, "split-precision GEMM needs finite inputs");
+    const double mra = hs.max_a, mrb = hs.max_b;
     if (s_use == 0) {
       const double target = target_err > 0 ? target_err : 1e-14;
       s_use = kMaxSlices;

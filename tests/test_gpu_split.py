@@ -199,3 +199,34 @@ def test_crt_exact_on_integers():
     torch.cuda.synchronize()
     assert info.int_bits >= 10  # 1000 < 2^10 fits the integer width, so scaling is exact
     assert np.array_equal(dC.cpu().numpy(), A @ B)
+
+
+def test_int8_level_configs_bit_identical():
+    """Every int8 level-kernel configuration computes the same exact integer residue GEMMs,
+    so the CRT result is the same bits whichever runs the large-problem path (m = n = 3000
+    gives 144 tiles of 256 x 256, enough for the large-problem kernel); the default's
+    result is within the FP64 target."""
+    import torch
+    m, k, n = 3000, 1000, 3000
+    dA = torch.empty((m, k), dtype=torch.float64, device="cuda")
+    dB = torch.empty((k, n), dtype=torch.float64, device="cuda")
+    phpc.fill_device(dA, phpc.DIST_WIDE, 41)
+    phpc.fill_device(dB, phpc.DIST_UNIFORM, 42)
+    results = {}
+    cfg = 0
+    try:
+        while phpc.lib().phpc_split_set_config(cfg) == 0:
+            dC = torch.zeros((m, n), dtype=torch.float64, device="cuda")
+            phpc.dgemm_split_device(dA, dB, dC, slices=0, target=1e-14, method=phpc.SPLIT_CRT)
+            torch.cuda.synchronize()
+            results[cfg] = dC
+            cfg += 1
+    finally:
+        phpc.lib().phpc_split_set_config(8)
+    assert len(results) >= 13
+    ref = results[8]
+    for c, dC in results.items():
+        assert torch.equal(dC.view(torch.int64), ref.view(torch.int64)), c
+    rng = np.random.default_rng(6)
+    cw, _ = phpc.check_sampled(dA, dB, ref, rng.integers(0, m, 512), rng.integers(0, n, 512))
+    assert cw <= 1e-14
