@@ -613,7 +613,7 @@ def main():
         cw, nw = phpc.check_sampled(A, B, C, rows, cols)
         i = info[0]
         tf = flops / (t_ms * 1e-3) / 1e12
-        kp = (N + 63) // 64 * 64
+        kp = (N + 127) // 128 * 128
         lvl_avg = lvl_ms / max(1, lvl_n)
         split[tag] = {"tflops_fp64_equiv": round(tf, 3), "ms": round(t_ms, 3), "vs_plain": round(tf / value, 3),
                       "moduli": i.slices, "int_bits": i.int_bits, "error_estimate": i.error_estimate,
@@ -713,7 +713,7 @@ def main():
             "unit": "TOP/s",
             "frac": round(uni["int8_kernel_tops"] / 5000.0, 4),
             "kernel_ms_avg": uni["int8_kernel_ms_avg"],
-            "ops_per_launch": 2.0 * N * N * ((N + 63) // 64 * 64),
+            "ops_per_launch": 2.0 * N * N * ((N + 127) // 128 * 128),
             "share_of_split_time": uni["int8_share_of_time"],
         },
         "split": split,

@@ -249,6 +249,11 @@ struct SharedWorkspace {
   }
 };
 
+// Bytes of k per residue / slice plane row of the split GEMM: k rounded up to 128, so
+// that every level range is a whole number of the level kernels' k slabs (64 or 128
+// bytes) and the 16-byte direct-to-LDS loads never straddle a plane.
+inline long phpc_split_kp(long k) { return (k + 127) / 128 * 128; }
+
 #include "phpc_split.h"
 // Split-precision GEMM with an explicit workspace and method (the public entry point
 // uses a process-wide workspace and the method set by phpc_split_set_method).

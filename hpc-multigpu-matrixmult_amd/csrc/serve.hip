@@ -346,7 +346,7 @@ extern "C" int phpc_server_create(size_t budget_bytes, int slots, int max_dim, i
   srv->max_dim = max_dim;
   // per-slot split workspace for max_dim^3 at up to 20 moduli: two int8 operand copies,
   // 20 residue planes, metadata
-  const size_t kp = ((size_t)max_dim + 63) / 64 * 64;
+  const size_t kp = (size_t)phpc_split_kp(max_dim);
   const size_t desc_bytes = (size_t)batch_max * std::max(sizeof(GemmDesc), sizeof(F32Desc));
   int rc = PHPC_OK;
   for (int i = 0; i < slots && rc == PHPC_OK; ++i) {

@@ -41,8 +41,9 @@ typedef double __attribute__((address_space(1))) gdouble;
 constexpr uint32_t kOOB = 0x80000000u;
 constexpr int kMaxSlices = 12;
 constexpr long kMaxExactTerms = 131072;  // products per int32 accumulation (127^2 * 2^17 < 2^31)
-// residues reach -128: 128^2 * 131008 < 2^31 (131008 = 2^17 - 64, a whole number of slabs)
-constexpr long kMaxExactTermsCrt = 131008;
+// residues reach -128: 128^2 * 130944 < 2^31 (130944 = 2^17 - 128, a whole number of
+// 128-byte slabs)
+constexpr long kMaxExactTermsCrt = 130944;
 // Widest integer operand of the CRT scheme: residues_of cuts |x| < 2^72 into nine bytes,
 // and the scaled operands satisfy |x| <= 2^p.
 constexpr int kMaxIntBits = 71;
@@ -592,12 +593,12 @@ __global__ __launch_bounds__(256) void bf16_level_128x128_w4(LevelArgs a) {
 
 typedef void (*LevelFn)(LevelArgs);
 struct I8Config {
-  int bm, bn, threads;
+  int bm, bn, bk, threads;
   size_t lds;
   LevelFn fn;
 };
 #define I8CFG(BM_, BN_, WM_, WN_, BK_, ST_, FN_) \
-  I8Config { BM_, BN_, WM_ *WN_ * 64, (size_t)(ST_)*I8Cfg<BM_, BN_, WM_, WN_, BK_, ST_>::STAGE_BYTES, FN_ }
+  I8Config { BM_, BN_, BK_, WM_ *WN_ * 64, (size_t)(ST_)*I8Cfg<BM_, BN_, WM_, WN_, BK_, ST_>::STAGE_BYTES, FN_ }
 const I8Config kI8Configs[] = {
     I8CFG(256, 256, 2, 4, 64, 3, i8_level_256x256_w8),            // 0: 8 waves (wave tile 128x64), 3 stages
     I8CFG(128, 128, 2, 2, 64, 4, i8_level_128x128_w4),            // 1: 4 waves, 64x64 wave tiles (small problems)
@@ -646,6 +647,11 @@ void init_locked() {
 }
 
 int launch_with(const I8Config &c, int occ, const LevelArgs &a, hipStream_t s) {
+  // the kernels walk k in whole slabs of c.bk bytes: a range that is not a whole number
+  // of slabs, or starts inside one, would lose its tail
+  if (a.kl <= 0 || a.kl % c.bk != 0 || a.a0 % 16 != 0 || a.b0 % 16 != 0 ||
+      (a.seg1 | a.seg2 | a.segb1 | a.segb2) % c.bk != 0)
+    return phpc_set_error(PHPC_ERR_ARG, "level range of %d bytes is not whole %d-byte slabs", a.kl, c.bk);
   const long tiles = (long)((a.m + c.bm - 1) / c.bm) * ((a.n + c.bn - 1) / c.bn);
   long G = (long)occ * phpc_device().cus;
   if (G > tiles) G = tiles;
@@ -1158,7 +1164,7 @@ int phpc_dgemm_split_ws(const double *dA, long lda, const double *dB, long ldb, 
     return phpc_set_error(PHPC_ERR_ARG, "moduli must be 0 (auto) or 2..%d", kMaxMods);
   if (m == 0 || n == 0 || k == 0) return phpc_set_error(PHPC_OK, nullptr);
   hipStream_t s = (hipStream_t)stream_;
-  const long kp = ((long)k + 63) / 64 * 64;
+  const long kp = phpc_split_kp(k);
 
   // scratch: exponents, ratios, column stats (slices and the FP64 accumulator below)
   const size_t small = split_meta_bytes(m, n);

@@ -143,10 +143,11 @@ def test_split_crt_config_sizes(N, dist):
     assert torch.equal(i64(dC), i64(dC2))
 
 
-def f32_sampled_ok(A, B, C, count, seed, C0=None):
+def f32_sampled_ok(A, B, C, count, seed, C0=None, extra=0.0):
     """C (FP32, on the device) against float64 dot products of the FP32 inputs on `count`
-    sampled entries, within (k + 2) 2^-24 (|C0| + |A||B|). Computed on the device with
-    elementwise products (FP32 x FP32 is exact in FP64)."""
+    sampled entries, within ((k + 2) 2^-24 + extra) (|C0| + |A||B|); extra = 2^-23 for the
+    3 x BF16 path's dropped slice terms. Computed on the device with elementwise products
+    (FP32 x FP32 is exact in FP64)."""
     import torch
     m, k = A.shape
     n = B.shape[1]
@@ -161,7 +162,7 @@ def f32_sampled_ok(A, B, C, count, seed, C0=None):
         ref = ref + C0[r, c].double()
         mag = mag + C0[r, c].double().abs()
     err = (C[r, c].double() - ref).abs()
-    bound = (k + 2) * 2.0 ** -24 * mag
+    bound = ((k + 2) * 2.0 ** -24 + extra) * mag
     return bool(torch.all(err <= bound)), float((err / mag.clamp_min(1e-300)).max())
 
 
@@ -283,7 +284,8 @@ def test_serving_poisson_stream_config_law():
             cw, _ = phpc.check_sampled(A, B, C, rows, cols)
             assert cw <= 1e-14, (i, n, cw)
         else:
-            ok, rel = f32_sampled_ok(A, B, C, 64, i)
+            bf16x3 = n * n >= 2048 * 2048 and ((n + 127) // 128) ** 2 >= 256
+            ok, rel = f32_sampled_ok(A, B, C, 64, i, extra=2.0 ** -23 if bf16x3 else 0.0)
             assert ok, (i, n, rel)
         del A, B, C
         outs[i] = None

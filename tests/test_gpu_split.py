@@ -205,9 +205,10 @@ def test_int8_level_configs_bit_identical():
     """Every int8 level-kernel configuration computes the same exact integer residue GEMMs,
     so the CRT result is the same bits whichever runs the large-problem path (m = n = 3000
     gives 144 tiles of 256 x 256, enough for the large-problem kernel); the default's
-    result is within the FP64 target."""
+    result is within the FP64 target. k = 900 is not a multiple of the 128-byte slab of
+    the large-problem kernels: the planes are padded to whole slabs."""
     import torch
-    m, k, n = 3000, 1000, 3000
+    m, k, n = 3000, 900, 3000
     dA = torch.empty((m, k), dtype=torch.float64, device="cuda")
     dB = torch.empty((k, n), dtype=torch.float64, device="cuda")
     phpc.fill_device(dA, phpc.DIST_WIDE, 41)
@@ -230,3 +231,20 @@ def test_int8_level_configs_bit_identical():
     rng = np.random.default_rng(6)
     cw, _ = phpc.check_sampled(dA, dB, ref, rng.integers(0, m, 512), rng.integers(0, n, 512))
     assert cw <= 1e-14
+
+
+@pytest.mark.parametrize("k,L", [(300, 6), (40, 4), (129, 5)])
+def test_crt_large_problem_kernel_ragged_k_matches_restatement(k, L):
+    """m = n = 2900 (144 tiles of 256 x 256) runs the residue GEMMs on the large-problem
+    kernel, which walks k in 128-byte slabs; k = 300, 40 and 129 leave a partial slab that
+    the zero padding of the planes must fill. The result equals the oracle's restatement
+    bit for bit."""
+    import torch
+    m = n = 2900
+    print(f"crt large-kernel m=n={m} k={k} L={L}", flush=True)
+    A, B, C0 = O.uniform((m, k), 51), O.wide((k, n), 52), O.uniform((m, n), 53)
+    dC = dev(C0)
+    info = phpc.dgemm_split_device(dev(A), dev(B), dC, slices=L, method=phpc.SPLIT_CRT)
+    torch.cuda.synchronize()
+    want = O.crt_gemm(A, B, C0.copy(), info.int_bits, L)
+    assert np.array_equal(bits(dC.cpu().numpy()), bits(want))
