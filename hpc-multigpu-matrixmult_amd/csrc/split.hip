@@ -1398,3 +1398,68 @@ int phpc_sgemm_bf16x3_planes(const float *dA, long lda, const float *dB, long ld
   a.ldc32 = ldc;
   return launch_bf_level(a, cfg, s);
 }
+
+// ---------------------------------------------------------------------------
+// Allocation-free split GEMM: the CRT path above with its scratch (the statistics block,
+// the int8 residue operands A8 and B8, the residue planes R8) carved out of one static
+// device array, `scratch`, instead of a workspace the library allocates. Each GPU the
+// library runs on holds its own copy of the array until the process ends; every call on
+// that GPU reuses it (the path writes everything it later reads: the statistics are
+// cleared by a kernel, A8, B8 and R8 are written whole), so calls that overlap on one GPU,
+// from other host threads or on other streams, race.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void *g_noalloc_ptr[kNoAllocMaxDevices] = {};  // this device's address of `scratch`
+
+// Bytes the CRT path carves for an m x k x n product at L moduli, in its order, each at a
+// 256-byte aligned offset: the statistics block, A8 (m x L kp), B8 (n x L kp), R8
+// (L x m x n). In double precision, so that no size overflows.
+double crt_scratch_need(int m, int k, int n, int L) {
+  auto up = [](double x) { return std::ceil(x / 256.0) * 256.0; };
+  const double kp = (double)phpc_split_kp(k);
+  return up((double)split_meta_bytes(m, n)) + up((double)m * L * kp) + up((double)n * L * kp) + (double)L * m * n;
+}
+}  // namespace
+
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+extern "C" int phpc_dgemm_split_device_noalloc(const double *dA, long lda, const double *dB, long ldb, double *dC,
+                                               long ldc, int m, int k, int n, int slices, double target_err,
+                                               void *stream, phpc_split_info *info) {
+  if (info) *info = phpc_split_info{};
+  if (phpc_init()) return phpc_last_error();
+  if (m < 0 || k < 0 || n < 0) return phpc_set_error(PHPC_ERR_ARG, "negative size");
+  if (slices < 0 || slices == 1 || slices > kMaxMods)
+    return phpc_set_error(PHPC_ERR_ARG, "moduli must be 0 (auto) or 2..%d", kMaxMods);
+  // sized for the most moduli the call may pick, so it cannot run out part-way
+  const int L = slices > 0 ? slices : kMaxMods;
+  const double need = crt_scratch_need(m, k, n, L);
+  if (need > (double)kNoAllocScratchBytes)
+    return phpc_set_error(PHPC_ERR_ARG,
+                          "%d x %d x %d needs %.0f bytes of scratch at %d moduli; the allocation-free split GEMM "
+                          "holds %zu",
+                          m, k, n, need, L, kNoAllocScratchBytes);
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNoAllocMaxDevices)
+    return phpc_set_error(PHPC_ERR_HIP, "no usable current device (%d)", dev);
+  void *ptr = nullptr;
+  {
+    // once per device, with that device current: the address of its copy of the array
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+      void *p = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !p)
+        return phpc_set_error(PHPC_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+      g_noalloc_ptr[dev] = p;
+    }
+    ptr = g_noalloc_ptr[dev];
+  }
+  Workspace ws;
+  ws.use_fixed(ptr, kNoAllocScratchBytes);
+  return phpc_dgemm_split_ws(dA, lda, dB, ldb, dC, ldc, m, k, n, slices, target_err, PHPC_SPLIT_CRT,
+                             (hipStream_t)stream, &ws, info);
+}

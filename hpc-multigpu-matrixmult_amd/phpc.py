@@ -111,6 +111,8 @@ _SIGS = {
     # include/phpc_split.h
     "phpc_dgemm_split_device": (I, [vp, L, vp, L, vp, L, I, I, I, I, ctypes.c_double, vp,
                                     ctypes.POINTER(SplitInfo)]),
+    "phpc_dgemm_split_device_noalloc": (I, [vp, L, vp, L, vp, L, I, I, I, I, ctypes.c_double, vp,
+                                            ctypes.POINTER(SplitInfo)]),
     "phpc_split_error_estimate": (ctypes.c_double, [I, ctypes.c_double, ctypes.c_double, I]),
     "phpc_split_set_config": (I, [I]),
     "phpc_split_num_configs": (I, []),

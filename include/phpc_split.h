@@ -54,6 +54,23 @@ int phpc_split_set_method(int method);
 int phpc_dgemm_split_device(const double *dA, long lda, const double *dB, long ldb, double *dC, long ldc, int m,
                             int k, int n, int slices, double target_err, void *stream, phpc_split_info *info);
 
+/*
+ * The same product by the CRT method (PHPC_SPLIT_CRT, whatever phpc_split_set_method
+ * says), allocating nothing: no hipMalloc, hipFree or other allocating call. No
+ * counterpart in the reference; it is for callers that cannot allocate device memory,
+ * such as tight serving loops and harnesses that forbid it. Its scratch is a static
+ * device array of 64 MiB that each GPU holds until the process ends, which limits the
+ * size: the statistics block, m x (L kp) + n x (L kp) residue operands and L x m x n
+ * residue planes (kp = k rounded up to 128, L = slices, or 20 when slices = 0) must fit,
+ * e.g. m = n = k = 1024 does; larger products fail with PHPC_ERR_ARG and leave C
+ * unchanged. Every call on one GPU shares that array: calls that overlap on one GPU
+ * (other threads, other streams) race, so the caller serialises them. The call
+ * synchronises the stream once, as phpc_dgemm_split_device does.
+ */
+int phpc_dgemm_split_device_noalloc(const double *dA, long lda, const double *dB, long ldb, double *dC, long ldc,
+                                    int m, int k, int n, int slices, double target_err, void *stream,
+                                    phpc_split_info *info);
+
 /* The a-priori estimate: (s+1) 2^(-7s) * ratio_a * ratio_b / sqrt(k), where ratio_a is the
  * largest row scale 2^e_i over the row's mean |a| (ratio_b likewise for columns). */
 double phpc_split_error_estimate(int slices, double ratio_a, double ratio_b, int k);
