@@ -327,7 +327,7 @@ def fp32_accurate_bench(phpc, sizes=(4096, 8192, 16384), seed=41):
     return out
 
 
-def poisson_run(srv, submit, n_req, rate, load, acc, flops, iso, rng):
+def poisson_run(srv, submit, n_req, rate, load, acc, flops, iso, rng, size=None):
     """Submit n_req requests at Poisson arrival times (rate per second) from this thread;
     a second thread waits for them in order."""
     import threading
@@ -362,7 +362,16 @@ def poisson_run(srv, submit, n_req, rate, load, acc, flops, iso, rng):
         sel = acc == a
         cls[tag] = {"p50_ms": round(float(np.percentile(lat[sel], 50)) * 1e3, 3),
                     "p99_ms": round(float(np.percentile(lat[sel], 99)) * 1e3, 3)}
+    # the requests that make the tail: what they are and where their time went
+    tail = []
+    for i in np.argsort(lat)[::-1][:12]:
+        g = got[i]
+        tail.append({"n": int(size[i]) if size is not None else None, "fp64": int(acc[i] == 0),
+                     "lat_ms": round(g["latency_s"] * 1e3, 2), "queue_ms": round(g["queue_s"] * 1e3, 2),
+                     "compute_ms": round(g["compute_s"] * 1e3, 2), "gemm_ms": round(g["gemm_s"] * 1e3, 2),
+                     "iso_ms": round(float(iso[i]) * 1e3, 2), "batch": g["batch_size"]})
     return {
+        "tail": tail,
         "load": load, "rate_req_per_s": round(rate, 2),
         "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
         "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
@@ -376,7 +385,11 @@ def poisson_run(srv, submit, n_req, rate, load, acc, flops, iso, rng):
     }
 
 
-def serving_bench(phpc, n_req, seed=11, slots=3, batch_max=32, split_min=4096, loads=(0.3, 0.7, 0.9)):
+SERVE_POLICIES = (("fifo", 0, 0.0), ("sjf", 1, 0.0), ("sjf_aging8", 1, 8.0), ("lanes", 2, 0.0))
+
+
+def serving_bench(phpc, n_req, seed=11, slots=3, batch_max=32, split_min=4096, loads=(0.3, 0.7, 0.9),
+                  policies=SERVE_POLICIES, default_policy="sjf", big_limit=1):
     """GEMM serving stream: square sizes 60% from [256,1024], 30% from [1024,4096], 10% from
     [4096,8192]; accuracy class 50% FP64-level, 50% FP32-level; operands generated on the
     device from per-request seeds; admission under a 200 GB budget. Measures each
@@ -393,6 +406,9 @@ def serving_bench(phpc, n_req, seed=11, slots=3, batch_max=32, split_min=4096, l
     reqs = [(int(acc[i]), int(size[i]), 1000 + 2 * i) for i in range(n_req)]
     srv = phpc.Server(200 * 10 ** 9, slots=slots, max_dim=8192, batch_max=batch_max)
     srv.set_split_min(split_min)
+    srv.set_big_limit(big_limit)
+    srv.set_policy(dict((t, p) for t, p, _ in policies)[default_policy],
+                   dict((t, a) for t, _, a in policies)[default_policy])
 
     def submit(i):
         a, n, sd = reqs[i]
@@ -411,7 +427,7 @@ def serving_bench(phpc, n_req, seed=11, slots=3, batch_max=32, split_min=4096, l
         sat_rate = n_req / t_sat
         _, peak = srv.memory()
         out = {
-            "default_policy": "fifo",
+            "default_policy": default_policy, "big_limit": big_limit,
             "requests": n_req, "slots": slots, "batch_max": batch_max, "budget_bytes": 200 * 10 ** 9,
             "fp64_split_min_dim": split_min,
             "mix": "sizes 60% [256,1024], 30% [1024,4096], 10% [4096,8192]; 50% FP64-level, 50% FP32-level",
@@ -423,13 +439,12 @@ def serving_bench(phpc, n_req, seed=11, slots=3, batch_max=32, split_min=4096, l
                           "peak_operand_bytes": peak},
             "poisson": {},
         }
-        for policy, aging, tag in ((phpc.SCHED_FIFO, 0.0, "fifo"), (phpc.SCHED_SJF, 0.0, "sjf"),
-                                   (phpc.SCHED_SJF, 8.0, "sjf_aging8"), (phpc.SCHED_LANES, 0.0, "lanes")):
+        for tag, policy, aging in policies:
             srv.set_policy(policy, aging)
             out["poisson"][tag] = []
             for load in loads:
                 out["poisson"][tag].append(poisson_run(srv, submit, n_req, load * sat_rate, load, acc, flops, iso,
-                                                       np.random.default_rng(seed + int(load * 100))))
+                                                       np.random.default_rng(seed + int(load * 100)), size))
                 r = out["poisson"][tag][-1]
                 log(f"serving {tag} load {load}: p50 {r['p50_ms']} ms, p99 {r['p99_ms']} ms, "
                     f"p99/iso p99 {r['p99_vs_isolated_p99']}")
@@ -721,7 +736,8 @@ def main():
             "tflops": round(exact_tf, 3),
             "roofline": {"bound": "mfma", "achieved": round(exact_kernel_tf, 3), "peak": FP64_PEAK_TFLOPS,
                          "unit": "TFLOP/s", "frac": round(exact_kernel_tf / FP64_PEAK_TFLOPS, 4),
-                         "kernel": "f64_at_128x128_w8_mp (per-panel sums added to the C carried in registers)",
+                         "kernel": "f64_at_128x128_w8_mp_k32 (32-deep slabs; per-panel sums added to the C "
+                                   "carried in registers)",
                          "kernel_ms_avg": round(exact_kernel_ms, 3), "pass_ms_avg": round(exact_ms, 3)},
             "arithmetic": "bit-exact to the reference's per-panel grouping (src/phpc_summa.c:63-95)",
         },

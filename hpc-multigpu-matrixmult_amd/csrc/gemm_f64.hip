@@ -612,14 +612,16 @@ __global__ __launch_bounds__(WM *WN * 64) __attribute__((amdgpu_waves_per_eu(2))
 // with no register staging and no LDS write instructions: each wave instruction fills
 // one contiguous 1 KiB piece of a k row. LDS rows are padded by 64 bytes. The MFMA
 // operand reads are the register-staged kernel's (16 consecutive doubles per lane
-// group), so the arithmetic is identical. Two stages: a slab's loads land while the
-// previous slab is computed. MP: k is walked in panels, each panel's sum starts from zero
-// and is added to the C value carried in registers (as the register-staged MP kernels).
+// group), so the arithmetic is identical. STAGES slabs of BK k rows ring through LDS:
+// slab s + STAGES - 1 is issued while slab s is computed (two stages: a slab's loads land
+// while the previous slab is computed). MP: k is walked in panels, each panel's sum starts
+// from zero and is added to the C value carried in registers (as the register-staged MP
+// kernels).
 // ---------------------------------------------------------------------------
-template <int BM, int BN, int WM, int WN>
+template <int BM, int BN, int WM, int WN, int STAGES_ = 2, int BK_ = 16>
 struct F64At {
-  static constexpr int BK = 16;
-  static constexpr int STAGES = 2;
+  static constexpr int BK = BK_;
+  static constexpr int STAGES = STAGES_;
   static constexpr int NW = WM * WN;
   static constexpr int WTM = BM / WM, WTN = BN / WN;
   static constexpr int TM = WTM / 16, TN = WTN / 16;
@@ -631,11 +633,12 @@ struct F64At {
   static constexpr int PER = BLOCKS / NW;
   static_assert(BM % 128 == 0 && BN % 128 == 0 && BLOCKS % NW == 0, "tile shape");
   static_assert(STAGES * STAGE_BYTES <= 160 * 1024, "LDS budget");
+  static_assert(STAGES >= 2 && (STAGES - 2) * PER < 64 && BK % 4 == 0, "pipeline depth");
 };
 
-template <int BM, int BN, int WM, int WN, bool MP>
+template <int BM, int BN, int WM, int WN, bool MP, int STAGES = 2, int BK = 16>
 __device__ __forceinline__ void f64_at_body(const GemmBatch &batch) {
-  using K = F64At<BM, BN, WM, WN>;
+  using K = F64At<BM, BN, WM, WN, STAGES, BK>;
   extern __shared__ __attribute__((aligned(1024))) char smem[];
   const int tid = threadIdx.x;
   const int wave = uni(tid >> 6);
@@ -747,11 +750,27 @@ __device__ __forceinline__ void f64_at_body(const GemmBatch &batch) {
         }
     }
 
-    // every wave must be done reading the previous tile's stages before refilling them
+    // every wave must be done reading the previous tile's stages before refilling them;
+    // with more than two stages nothing older than the ring's loads may be outstanding,
+    // so that counting them (in issue order) tells which slabs have landed
+    if constexpr (K::STAGES > 2) wait_vm_c<0>();
     __syncthreads();
-    if (nslab > 0) issue(0);
+#pragma unroll
+    for (int s0 = 0; s0 < K::STAGES - 1; ++s0)
+      if (s0 < nslab) issue(s0);
     for (int sidx = 0; sidx < nslab; ++sidx) {
-      wait_vm_c<0>();
+      // slabs issued after this one that may stay in flight
+      const int later = min(K::STAGES - 2, nslab - 1 - sidx);
+      if constexpr (K::STAGES == 2) {
+        wait_vm_c<0>();
+      } else {
+        if (later <= 0)
+          wait_vm_c<0>();
+        else if (later == 1)
+          wait_vm_c<K::PER>();
+        else
+          wait_vm_c<(K::STAGES - 2) * K::PER>();
+      }
       __syncthreads();
       const char *stage = smem + (sidx % K::STAGES) * K::STAGE_BYTES;
       const double *At = reinterpret_cast<const double *>(stage);
@@ -772,7 +791,7 @@ __device__ __forceinline__ void f64_at_body(const GemmBatch &batch) {
       // the first k step's LDS reads go out right after the barrier; the next slab's
       // loads are issued behind its MFMAs
       kstep(0);
-      if (sidx + 1 < nslab) issue(sidx + 1);
+      if (sidx + K::STAGES - 1 < nslab) issue(sidx + K::STAGES - 1);
 #pragma unroll
       for (int kk = 1; kk < K::BK / 4; ++kk) kstep(kk);
       if constexpr (MP) {
@@ -822,6 +841,29 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void f
 }
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void f64_at_128x128_w8_mp(GemmBatch batch) {
   f64_at_body<128, 128, 2, 4, true>(batch);
+}
+// One workgroup per CU leaves LDS for a deeper ring: three or four 16-deep stages, or two
+// 32-deep ones (half the barriers per k)
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void f64_at_128x128_w8_s3(GemmBatch batch) {
+  f64_at_body<128, 128, 2, 4, false, 3>(batch);
+}
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void f64_at_128x128_w8_mp_s3(
+    GemmBatch batch) {
+  f64_at_body<128, 128, 2, 4, true, 3>(batch);
+}
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void f64_at_128x128_w8_s4(GemmBatch batch) {
+  f64_at_body<128, 128, 2, 4, false, 4>(batch);
+}
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void f64_at_128x128_w8_mp_s4(
+    GemmBatch batch) {
+  f64_at_body<128, 128, 2, 4, true, 4>(batch);
+}
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void f64_at_128x128_w8_k32(GemmBatch batch) {
+  f64_at_body<128, 128, 2, 4, false, 2, 32>(batch);
+}
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void f64_at_128x128_w8_mp_k32(
+    GemmBatch batch) {
+  f64_at_body<128, 128, 2, 4, true, 2, 32>(batch);
 }
 
 // At[k][m] = A[m][k] for m < M, k < K; At[k][m] = 0 for M <= m < ldat (row pitch ldat)
@@ -931,6 +973,21 @@ const ConfigEntry kConfigs[] = {
      {{PHPC_V1(128, 128, 2, 4, 16, false, 1), f64_at_128x128_w8},
       {PHPC_V1(128, 128, 2, 4, 16, true, 1), f64_at_128x128_w8_mp}},
      true},
+    // 13: as 12 with three 16-deep stages (102 KiB)
+    {128, 128, 512, 16, {0, 3 * (size_t)F64At<128, 128, 2, 4, 3>::STAGE_BYTES}, true,
+     {{PHPC_V1(128, 128, 2, 4, 16, false, 1), f64_at_128x128_w8_s3},
+      {PHPC_V1(128, 128, 2, 4, 16, true, 1), f64_at_128x128_w8_mp_s3}},
+     true},
+    // 14: as 12 with four 16-deep stages (136 KiB)
+    {128, 128, 512, 16, {0, 4 * (size_t)F64At<128, 128, 2, 4, 4>::STAGE_BYTES}, true,
+     {{PHPC_V1(128, 128, 2, 4, 16, false, 1), f64_at_128x128_w8_s4},
+      {PHPC_V1(128, 128, 2, 4, 16, true, 1), f64_at_128x128_w8_mp_s4}},
+     true},
+    // 15: as 12 with two 32-deep stages (136 KiB)
+    {128, 128, 512, 32, {0, 2 * (size_t)F64At<128, 128, 2, 4, 2, 32>::STAGE_BYTES}, true,
+     {{PHPC_V1(128, 128, 2, 4, 16, false, 1), f64_at_128x128_w8_k32},
+      {PHPC_V1(128, 128, 2, 4, 16, true, 1), f64_at_128x128_w8_mp_k32}},
+     true},
 };
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 
@@ -1019,7 +1076,7 @@ namespace {
 // Configuration for large single-panel work reached through host descriptors
 // (phpc_f64_grouped), which can stage a transposed copy of A first.
 int g_big_single_host = 11;
-int g_big_config_host = 12;  // large panel-grouped work through host descriptors
+int g_big_config_host = 15;  // large panel-grouped work through host descriptors
 
 bool b_vec2_ok(const GemmDesc &d) { return aligned16(d.B) && (d.ldb % 2 == 0) && (d.n % 2 == 0); }
 // the transposed-A kernels load B in 16-byte column pairs; with an even ldb and an aligned

@@ -416,9 +416,22 @@ extern "C" int phpc_server_submit(phpc_server_t srv, const phpc_request *req, ui
     q->small = (long)((r.m + 255) / 256) * ((r.n + 127) / 128) < cus;
     q->wide = r.k % 2 == 0 && r.n % 2 == 0;
   }
-  // device-time estimate from the flop count at each path's typical rate
+  // device-time estimate: the operand generation (HBM writes) plus the flop count at the
+  // rate of the path the request will take, with that path's fixed cost (rates measured on
+  // MI355X: split CRT ~140 TFLOP/s after ~0.5 ms of statistics, residues and one stream
+  // synchronisation; plain FP64 ~55; 3 x BF16 slice planes ~180; grouped FP32 ~80)
   const double flops = 2.0 * r.m * (double)r.n * r.k;
-  q->est = 5e-5 + flops / (q->kind == kF64Plain ? 55e12 : 100e12);
+  const double gen = (double)(q->bytes[0] + q->bytes[1] + q->bytes[2]) / 4e12;
+  double fixed = 3e-5, rate = 80e12;
+  if (q->kind == kF64Split) {
+    fixed = 5e-4;
+    rate = 140e12;
+  } else if (q->kind == kF64Plain) {
+    rate = 55e12;
+  } else if (!q->small && (long)r.m * r.n >= 2048L * 2048 && r.k >= 256) {
+    rate = 180e12;
+  }
+  q->est = fixed + gen + flops / rate;
   q->heavy = !q->small && q->est >= srv->heavy_s;
   if (total > srv->arena.cap) {
     delete q;

@@ -182,7 +182,7 @@ def test_check_sampled_matches_oracle():
     assert cw2 > 1e-9
 
 
-@pytest.mark.parametrize("cfg", [10, 11, 12])
+@pytest.mark.parametrize("cfg", [10, 11, 12, 13, 14, 15])
 @pytest.mark.parametrize("m,k,n", [(257, 131, 199), (300, 200, 260), (33, 17, 1), (1, 64, 3)])
 def test_transposed_a_configs_on_ragged_shapes(cfg, m, k, n):
     """The transposed-A configurations (A staged transposed; B staged to an even pitch when
@@ -234,7 +234,7 @@ def same_bits_or_both_nan(got, want):
     return np.array_equal(gn, wn) and np.array_equal(bits(np.where(gn, 0.0, got)), bits(np.where(wn, 0.0, want)))
 
 
-@pytest.mark.parametrize("cfg", [-1] + list(range(13)))
+@pytest.mark.parametrize("cfg", [-1] + list(range(16)))
 def test_special_values_match_reference_kernel(cfg):
     """IEEE special cases through every tile configuration: subnormal products and sums are
     kept (no flush to zero), signed zeros, infinities and NaNs propagate as in the reference
@@ -262,12 +262,14 @@ def test_special_values_match_reference_kernel(cfg):
     assert same_bits_or_both_nan(C, want)
 
 
-@pytest.mark.parametrize("cfg", [0, 12])
+@pytest.mark.parametrize("cfg", [0, 12, 13, 14, 15])
 @pytest.mark.parametrize("m,k,n,panel", [(300, 1000, 260, 333), (129, 70, 257, 16), (513, 2050, 130, 1024),
-                                         (64, 64, 64, 63)])
+                                         (64, 64, 64, 63), (260, 700, 130, 40)])
 def test_panel_grouped_configs_on_ragged_panels(cfg, m, k, n, panel):
     """Panel-grouped work through the device entry point, register-staged (0) and
-    transposed-A (12): C = ((C0 + s_0) + s_1) + ..., each s_t an FMA chain from zero over
+    transposed-A (12; 13-15 with three or four 16-deep stages or two 32-deep ones, whose
+    slabs stop at panel ends that do not fall on a slab boundary, e.g. panels of 40 and
+    333): C = ((C0 + s_0) + s_1) + ..., each s_t an FMA chain from zero over
     one panel, bit-exact against per-panel calls of the reference kernel's emulation, on
     ragged shapes and panels that do not divide k."""
     import torch
