@@ -353,22 +353,16 @@ __device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
   const int sb1 = BF ? uni(args.segb1) : 0, sb2 = BF ? uni(args.segb2) : 0;
 
   // LDS operand offsets per k step: lane reads 16 bytes of row (l % 32), chunk
-  // 2*kk + l/32, at its swizzled slot
-  int a_off[K::KSTEPS][K::TM], b_off[K::KSTEPS][K::TN];
+  // 2*kk + l/32, at its swizzled slot. The rows a lane reads are l % 32 plus multiples of
+  // 32, and the swizzle of a multiple of 32 is 0, so the slot depends on l and kk alone:
+  // fragment i of A sits at a_row + i*32*BK + cofs[kk] (B likewise), the constant part
+  // folding into the LDS instruction's offset.
+  const int sw = i8_swz<K::CPR, SD>(lane & 31);
+  const int a_row = (wm * K::WTM + (lane & 31)) * K::BK;
+  const int b_row = K::A_BYTES + (wn * K::WTN + (lane & 31)) * K::BK;
+  int cofs[K::KSTEPS];
 #pragma unroll
-  for (int kk = 0; kk < K::KSTEPS; ++kk) {
-    const int c = 2 * kk + (lane >> 5);
-#pragma unroll
-    for (int i = 0; i < K::TM; ++i) {
-      const int r = wm * K::WTM + i * 32 + (lane & 31);
-      a_off[kk][i] = r * K::BK + ((c ^ i8_swz<K::CPR, SD>(r)) << 4);
-    }
-#pragma unroll
-    for (int j = 0; j < K::TN; ++j) {
-      const int r = wn * K::WTN + j * 32 + (lane & 31);
-      b_off[kk][j] = K::A_BYTES + r * K::BK + ((c ^ i8_swz<K::CPR, SD>(r)) << 4);
-    }
-  }
+  for (int kk = 0; kk < K::KSTEPS; ++kk) cofs[kk] = ((2 * kk + (lane >> 5)) ^ sw) << 4;
 
   for (int round = 0;; ++round) {
     const int t = round * G + slot;
@@ -386,18 +380,23 @@ __device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
     const rsrc_t rB = make_rsrc(B8 + (long)col0 * ldb8, (long)min(n_rem, BN) * ldb8);
 
     // this wave's blocks: wave, wave + NW, ...; lane p of a block fills slot p % CPR of
-    // row p / CPR, i.e. chunk (p % CPR) ^ sw(row); offsets for k = 0 (kb in soffset)
-    uint32_t goff[K::PER];
-#pragma unroll
-    for (int u = 0; u < K::PER; ++u) {
-      const int blk = wave + u * K::NW;
-      const bool isA = blk < K::A_BLOCKS;
-      const int r = (isA ? blk : blk - K::A_BLOCKS) * K::ROWS_PER_BLOCK + lane / K::CPR;
-      const int c = (lane % K::CPR) ^ i8_swz<K::CPR, SD>(r);
-      const int rows = isA ? min(m_rem, BM) : min(n_rem, BN);
-      const long ld = isA ? lda8 : ldb8;
-      goff[u] = r < rows ? (uint32_t)((long)r * ld + c * 16) : kOOB;
-    }
+    // row p / CPR, i.e. chunk (p % CPR) ^ sw(row); offsets for k = 0 (kb in soffset).
+    // Block u of A (u < PA) covers rows (wave + u NW) RPB + lane / CPR: the row advances
+    // by NW RPB per u, a multiple of 32, so the swizzled slot is the same for every u and
+    // the offset is a0 + u * (NW RPB lda8) (B likewise, its blocks numbered after A's).
+    // Rows past the tile's last one (m_rem, n_rem) fall at or beyond the buffer's
+    // num_records (rows x ld), so they read zero.
+    constexpr int PA = K::A_BLOCKS / K::NW;
+    constexpr int RSTEP = K::NW * K::ROWS_PER_BLOCK;
+    static_assert(K::A_BLOCKS % K::NW == 0 && RSTEP % 32 == 0, "blocks split evenly between A and B");
+    const int lrow = lane / K::CPR;
+    const uint32_t ga0 =
+        (uint32_t)((long)(wave * K::ROWS_PER_BLOCK + lrow) * lda8 +
+                   (((lane % K::CPR) ^ i8_swz<K::CPR, SD>(wave * K::ROWS_PER_BLOCK + lrow)) << 4));
+    const uint32_t gb0 =
+        (uint32_t)((long)(wave * K::ROWS_PER_BLOCK + lrow) * ldb8 +
+                   (((lane % K::CPR) ^ i8_swz<K::CPR, SD>(wave * K::ROWS_PER_BLOCK + lrow)) << 4));
+    const uint32_t gstep_a = (uint32_t)(RSTEP * lda8), gstep_b = (uint32_t)(RSTEP * ldb8);
 
     // k byte offsets of slab sidx in A's and B's rows
     auto slab_k = [&](int sidx, int &ka, int &kbb) {
@@ -424,9 +423,11 @@ __device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
 #pragma unroll
       for (int u = 0; u < K::PER; ++u) {
         const int blk = wave + u * K::NW;
-        const bool isA = blk < K::A_BLOCKS;
-        // the range check applies to the lane offset (kOOB reads zero); k rides in soffset
-        __builtin_amdgcn_raw_ptr_buffer_load_lds(isA ? rA : rB, (lds_void *)(stage + blk * 1024), 16, goff[u],
+        const bool isA = u < PA;
+        // the range check applies to the lane offset (past num_records reads zero); k
+        // rides in soffset
+        const uint32_t go = isA ? ga0 + (uint32_t)u * gstep_a : gb0 + (uint32_t)(u - PA) * gstep_b;
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(isA ? rA : rB, (lds_void *)(stage + blk * 1024), 16, go,
                                                   isA ? ka : kbb, 0, 0);
       }
     };
@@ -461,9 +462,11 @@ __device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
       for (int kk = 0; kk < K::KSTEPS; ++kk) {
         i4v a[K::TM], b[K::TN];
 #pragma unroll
-        for (int i = 0; i < K::TM; ++i) a[i] = *reinterpret_cast<const i4v *>(stage + a_off[kk][i]);
+        for (int i = 0; i < K::TM; ++i)
+          a[i] = *reinterpret_cast<const i4v *>(stage + a_row + cofs[kk] + i * 32 * K::BK);
 #pragma unroll
-        for (int j = 0; j < K::TN; ++j) b[j] = *reinterpret_cast<const i4v *>(stage + b_off[kk][j]);
+        for (int j = 0; j < K::TN; ++j)
+          b[j] = *reinterpret_cast<const i4v *>(stage + b_row + cofs[kk] + j * 32 * K::BK);
 #pragma unroll
         for (int i = 0; i < K::TM; ++i)
 #pragma unroll
@@ -652,6 +655,10 @@ int launch_with(const I8Config &c, int occ, const LevelArgs &a, hipStream_t s) {
   if (a.kl <= 0 || a.kl % c.bk != 0 || a.a0 % 16 != 0 || a.b0 % 16 != 0 ||
       (a.seg1 | a.seg2 | a.segb1 | a.segb2) % c.bk != 0)
     return phpc_set_error(PHPC_ERR_ARG, "level range of %d bytes is not whole %d-byte slabs", a.kl, c.bk);
+  // buffer offsets are 32-bit: a tile's rows of A and B must span less than 2 GiB
+  if ((long)c.bm * a.lda8 >= 0x7fffffffL || (long)c.bn * a.ldb8 >= 0x7fffffffL)
+    return phpc_set_error(PHPC_ERR_ARG, "operand rows of %ld / %ld bytes too long for %d x %d tiles", a.lda8,
+                          a.ldb8, c.bm, c.bn);
   const long tiles = (long)((a.m + c.bm - 1) / c.bm) * ((a.n + c.bn - 1) / c.bn);
   long G = (long)occ * phpc_device().cus;
   if (G > tiles) G = tiles;
