@@ -736,8 +736,8 @@ def main():
             "tflops": round(exact_tf, 3),
             "roofline": {"bound": "mfma", "achieved": round(exact_kernel_tf, 3), "peak": FP64_PEAK_TFLOPS,
                          "unit": "TFLOP/s", "frac": round(exact_kernel_tf / FP64_PEAK_TFLOPS, 4),
-                         "kernel": "f64_at_128x128_w8_mp_k32 (32-deep slabs; per-panel sums added to the C "
-                                   "carried in registers)",
+                         "kernel": "f64_at_128x128_w16_mp_k32 (16 waves, 32-deep slabs; per-panel sums added "
+                                   "to the C carried in registers)",
                          "kernel_ms_avg": round(exact_kernel_ms, 3), "pass_ms_avg": round(exact_ms, 3)},
             "arithmetic": "bit-exact to the reference's per-panel grouping (src/phpc_summa.c:63-95)",
         },

@@ -660,10 +660,12 @@ int f32_at_run(const F32Batch &b, bool b4, int grid, hipStream_t stream) {
   int G = grid > 0 ? grid : occ_v * phpc_device().cus;
   if (G > b.total_tiles) G = b.total_tiles;
   if (G < 1) G = 1;
+  const int kt = phpc_ktime_begin(stream);
   if (b4)
     hipLaunchKernelGGL(f32_at_128x128_b4, dim3(G), dim3(256), kF32AtLds, stream, b);
   else
     hipLaunchKernelGGL(f32_at_128x128, dim3(G), dim3(256), kF32AtLds, stream, b);
+  phpc_ktime_end(kt, stream);
   PHPC_HIP(hipGetLastError());
   return PHPC_OK;
 }
@@ -887,7 +889,9 @@ int phpc_f32_launch(const F32Desc *descs_dev, const F32Desc *descs_host, int nde
   int G = grid > 0 ? grid : occ(id, vec4) * phpc_device().cus;
   if (G > total_tiles) G = total_tiles;
   if (G < 1) G = 1;
+  const int kt = phpc_ktime_begin(stream);
   hipLaunchKernelGGL(kF32[id].fn[vec4 ? 1 : 0], dim3(G), dim3(256), 0, stream, b);
+  phpc_ktime_end(kt, stream);
   PHPC_HIP(hipGetLastError());
   return PHPC_OK;
 }

@@ -865,6 +865,16 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void f
     GemmBatch batch) {
   f64_at_body<128, 128, 2, 4, true, 2, 32>(batch);
 }
+// 16 waves with 32x32 wave tiles: the carried C and the accumulators fit 4 waves per SIMD,
+// so each barrier has four waves per SIMD to hide behind (LDS reads: one per MFMA)
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void f64_at_128x128_w16_k32(
+    GemmBatch batch) {
+  f64_at_body<128, 128, 4, 4, false, 2, 32>(batch);
+}
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void f64_at_128x128_w16_mp_k32(
+    GemmBatch batch) {
+  f64_at_body<128, 128, 4, 4, true, 2, 32>(batch);
+}
 
 // At[k][m] = A[m][k] for m < M, k < K; At[k][m] = 0 for M <= m < ldat (row pitch ldat)
 __global__ __launch_bounds__(256) void f64_transpose_kernel(const double *__restrict__ A, long lda, int M, int Kd,
@@ -988,6 +998,15 @@ const ConfigEntry kConfigs[] = {
      {{PHPC_V1(128, 128, 2, 4, 16, false, 1), f64_at_128x128_w8_k32},
       {PHPC_V1(128, 128, 2, 4, 16, true, 1), f64_at_128x128_w8_mp_k32}},
      true},
+    // 16: as 15 with 16 waves (wave tile 32x32). There is no 1024-thread register-staged
+    //     kernel for the 8-byte slots; they are never launched (a transposed-A launch
+    //     needs 16-byte B rows, and phpc_f64_grouped switches to config 0 when A cannot be
+    //     staged), so they hold the same kernels, keeping every slot's shape 1024 threads.
+    {128, 128, 1024, 32, {2 * (size_t)F64At<128, 128, 4, 4, 2, 32>::STAGE_BYTES,
+                          2 * (size_t)F64At<128, 128, 4, 4, 2, 32>::STAGE_BYTES}, true,
+     {{f64_at_128x128_w16_k32, f64_at_128x128_w16_k32},
+      {f64_at_128x128_w16_mp_k32, f64_at_128x128_w16_mp_k32}},
+     true},
 };
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 
@@ -1076,7 +1095,7 @@ namespace {
 // Configuration for large single-panel work reached through host descriptors
 // (phpc_f64_grouped), which can stage a transposed copy of A first.
 int g_big_single_host = 11;
-int g_big_config_host = 15;  // large panel-grouped work through host descriptors
+int g_big_config_host = 16;  // large panel-grouped work through host descriptors
 
 bool b_vec2_ok(const GemmDesc &d) { return aligned16(d.B) && (d.ldb % 2 == 0) && (d.n % 2 == 0); }
 // the transposed-A kernels load B in 16-byte column pairs; with an even ldb and an aligned

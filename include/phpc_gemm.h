@@ -131,8 +131,8 @@ int phpc_check_sampled_host(const double *A, long lda, const double *B, long ldb
 
 /*
  * Device-event timing of the main GEMM kernel launches, for measurement: while on != 0,
- * every FP64 tile-kernel launch and every split-precision level-kernel launch (int8
- * residue / slice GEMMs, BF16 slice GEMMs) is bracketed by a pair of HIP events on the
+ * every FP64 and FP32 tile-kernel launch and every split-precision level-kernel launch
+ * (int8 residue / slice GEMMs, BF16 slice GEMMs) is bracketed by a pair of HIP events on the
  * stream it runs on; launches into a stream being captured into a graph are not timed.
  * phpc_kernel_timing() discards what was recorded; phpc_kernel_timing_read() waits for
  * the recorded launches, returns their summed duration (ms) and count, and clears them.

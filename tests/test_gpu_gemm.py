@@ -182,7 +182,7 @@ def test_check_sampled_matches_oracle():
     assert cw2 > 1e-9
 
 
-@pytest.mark.parametrize("cfg", [10, 11, 12, 13, 14, 15])
+@pytest.mark.parametrize("cfg", [10, 11, 12, 13, 14, 15, 16])
 @pytest.mark.parametrize("m,k,n", [(257, 131, 199), (300, 200, 260), (33, 17, 1), (1, 64, 3)])
 def test_transposed_a_configs_on_ragged_shapes(cfg, m, k, n):
     """The transposed-A configurations (A staged transposed; B staged to an even pitch when
@@ -234,7 +234,7 @@ def same_bits_or_both_nan(got, want):
     return np.array_equal(gn, wn) and np.array_equal(bits(np.where(gn, 0.0, got)), bits(np.where(wn, 0.0, want)))
 
 
-@pytest.mark.parametrize("cfg", [-1] + list(range(16)))
+@pytest.mark.parametrize("cfg", [-1] + list(range(17)))
 def test_special_values_match_reference_kernel(cfg):
     """IEEE special cases through every tile configuration: subnormal products and sums are
     kept (no flush to zero), signed zeros, infinities and NaNs propagate as in the reference
@@ -262,7 +262,7 @@ def test_special_values_match_reference_kernel(cfg):
     assert same_bits_or_both_nan(C, want)
 
 
-@pytest.mark.parametrize("cfg", [0, 12, 13, 14, 15])
+@pytest.mark.parametrize("cfg", [0, 12, 13, 14, 15, 16])
 @pytest.mark.parametrize("m,k,n,panel", [(300, 1000, 260, 333), (129, 70, 257, 16), (513, 2050, 130, 1024),
                                          (64, 64, 64, 63), (260, 700, 130, 40)])
 def test_panel_grouped_configs_on_ragged_panels(cfg, m, k, n, panel):

@@ -33,7 +33,7 @@ def test_summa_device_modes_match_oracle(p, q, N, panel):
         assert np.array_equal(bits(dC.cpu().numpy()), bits(want)), mode
 
 
-@pytest.mark.parametrize("cfg", [0, 3, 12, 13, 14, 15])
+@pytest.mark.parametrize("cfg", [0, 3, 12, 13, 14, 15, 16])
 @pytest.mark.parametrize("p,q,N,panel", [(4, 4, 512, 32), (2, 3, 257, 16), (3, 3, 300, 7), (2, 2, 256, 0)])
 def test_summa_fused_panel_configs_match_oracle(cfg, p, q, N, panel):
     """FUSED on each panel-grouped tile configuration, the transposed-A one (12) included,
