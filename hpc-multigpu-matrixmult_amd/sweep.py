@@ -10,7 +10,12 @@ speedup_cublas = t_iterative / cublas_time, efficiency = speedup / total_threads
 efficiency_kernel = speedup_kernel / total_threads, printed with six decimals.
 
     python sweep.py --sweep testB [--iterative csv/iterative.csv] [--extra "--mode chained"]
+    python sweep.py --sweep testB --run-iterative      # writes csv/iterative.csv first
     python sweep.py --runs "4 512 32 1 1,16 1024 32 2 2" --name mine
+
+--run-iterative produces the CPU baseline file the way the harness's sequential block does
+(scripts/tests.sh:36-44): bin/iterative.out once per matrix size of the sweep, its "n,seconds"
+lines gathered into csv/iterative.csv.
 """
 from __future__ import annotations
 
@@ -22,6 +27,7 @@ from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
 MAIN = HERE / "bin" / "main.out"
+ITERATIVE = HERE / "bin" / "iterative.out"
 
 # (P, N, T, GW, GH) per run and the sort key of the gathered file (scripts/tests.sh:45-100)
 SWEEPS = {
@@ -89,6 +95,21 @@ def read_iterative(path):
     return times
 
 
+def run_iterative(sizes, workdir=".", exe=ITERATIVE):
+    """bin/iterative.out for each size, its "n,seconds" lines into csv/iterative.csv (the
+    harness's sequential block, scripts/tests.sh:36-44); returns that file's path."""
+    (Path(workdir) / "csv").mkdir(parents=True, exist_ok=True)
+    lines = []
+    for n in sorted(set(sizes)):
+        r = subprocess.run([str(exe), str(n)], cwd=workdir, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"{exe} {n} failed ({r.returncode}): {r.stderr.strip()}")
+        lines.append(r.stdout.strip())
+    out = Path(workdir) / "csv" / "iterative.csv"
+    out.write_text("\n".join(lines) + "\n")
+    return out
+
+
 def sweep(runs, sort_key, name, workdir=".", iterative=None, extra=()):
     (Path(workdir) / "csv").mkdir(parents=True, exist_ok=True)
     rows = [run_one(*r, name, workdir, list(extra)) for r in runs]
@@ -106,6 +127,8 @@ def main(argv=None):
     ap.add_argument("--runs", help="comma-separated 'P N T GW GH' runs (instead of --sweep)")
     ap.add_argument("--name", help="output name (default: the sweep's)")
     ap.add_argument("--iterative", help="CPU baseline times, 'n,seconds' per line")
+    ap.add_argument("--run-iterative", action="store_true",
+                    help="first run bin/iterative.out for every size of the sweep into csv/iterative.csv")
     ap.add_argument("--extra", default="", help="extra main.out options, e.g. '--mode chained --dist uniform'")
     ap.add_argument("--workdir", default=".")
     a = ap.parse_args(argv)
@@ -119,6 +142,8 @@ def main(argv=None):
         key, name = "size", a.name or "sweep"
     else:
         ap.error("give --sweep or --runs")
+    if a.run_iterative:
+        a.iterative = str(run_iterative([r[1] for r in runs], a.workdir))
     it = read_iterative(a.iterative) if a.iterative else None
     out = sweep(runs, key, name, a.workdir, it, shlex.split(a.extra))
     print(out)

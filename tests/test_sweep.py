@@ -51,3 +51,17 @@ def test_gathered_csv_reads_like_the_reference_plots(tmp_path):
     assert abs(speedup[0] - 150.0) < 1e-9
     assert abs(results["time_cuda_gpu"][1] - 0.016) < 1e-12 and abs(results["time_cublas"][1] - 0.05) < 1e-12
     assert abs(results["speedup"][0] - 150.0) < 1e-9 and abs(results["efficiency_kernel"][1] - 0.183105) < 1e-9
+
+
+def test_run_iterative_writes_the_harness_baseline_file(tmp_path):
+    """--run-iterative: bin/iterative.out once per distinct size, "n,seconds" lines in
+    csv/iterative.csv, which read_iterative() then feeds to process()."""
+    out = sweep.run_iterative([48, 16, 48, 32], tmp_path)
+    assert out == tmp_path / "csv" / "iterative.csv"
+    lines = out.read_text().split()
+    assert [ln.split(",")[0] for ln in lines] == ["16", "32", "48"]
+    times = sweep.read_iterative(out)
+    assert sorted(times) == [16, 32, 48] and all(t >= 0.0 for t in times.values())
+    row = "48,4,1,1,1024,1024,0.001,0.0005,0.002"
+    got = sweep.process([row], {48: 1.0})[1].split(",")
+    assert got[9:12] == ["1000.000000", "2000.000000", "500.000000"]
