@@ -38,7 +38,6 @@ typedef __amdgpu_buffer_rsrc_t rsrc_t;
 typedef __attribute__((address_space(3))) void lds_void;
 typedef double __attribute__((address_space(1))) gdouble;
 
-constexpr uint32_t kOOB = 0x80000000u;
 constexpr int kMaxSlices = 12;
 constexpr long kMaxExactTerms = 131072;  // products per int32 accumulation (127^2 * 2^17 < 2^31)
 // residues reach -128: 128^2 * 130944 < 2^31 (130944 = 2^17 - 128, a whole number of
