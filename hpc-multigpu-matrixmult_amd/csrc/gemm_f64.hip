@@ -366,6 +366,15 @@ __device__ __forceinline__ void wait_vm_c() {
   static_assert(N >= 0 && N < 64, "vmcnt range");
   __builtin_amdgcn_s_waitcnt(0x0F70 | (N & 0xF) | (((N >> 4) & 3) << 14));
 }
+// Workgroup barrier of the slab rings. __syncthreads() brackets s_barrier with
+// workgroup-scope fences, and the release fence makes the compiler wait for every
+// outstanding vector-memory operation (vmcnt(0)) first: that drains the direct-to-LDS
+// loads of the slabs still in flight and leaves a ring one slab deep whatever its number
+// of stages. The rings wait for the loads they need themselves (wait_vm_c) and write no
+// LDS with ordinary stores, so the barrier only has to keep each wave's LDS reads of a
+// slab ahead of the loads that refill its buffer; the memory clobber keeps the compiler
+// from moving memory operations across it.
+__device__ __forceinline__ void slab_barrier() { asm volatile("s_barrier" ::: "memory"); }
 
 template <int BM, int BN_, int WM, int WN, int STAGES, bool MP>
 struct F64Lds {
@@ -552,7 +561,7 @@ __global__ __launch_bounds__(WM *WN * 64) __attribute__((amdgpu_waves_per_eu(2))
     };
 
     // every wave must be done reading the previous tile's stages before refilling them
-    __syncthreads();
+    slab_barrier();
 #pragma unroll
     for (int s0 = 0; s0 < STAGES - 1; ++s0)
       if (s0 < nslab) issue(s0);
@@ -560,7 +569,7 @@ __global__ __launch_bounds__(WM *WN * 64) __attribute__((amdgpu_waves_per_eu(2))
     for (int sidx = 0; sidx < nslab; ++sidx) {
       // loads of slab sidx are complete once at most the later slabs' are in flight
       wait_outstanding<K::PER_LO, K::PER_HI>(min(STAGES - 2, nslab - 1 - sidx), my_blocks);
-      __syncthreads();
+      slab_barrier();
       if (sidx + STAGES - 1 < nslab) issue(sidx + STAGES - 1);
       const int p = sidx / spp;
       const int j = sidx - p * spp;
@@ -754,7 +763,7 @@ __device__ __forceinline__ void f64_at_body(const GemmBatch &batch) {
     // with more than two stages nothing older than the ring's loads may be outstanding,
     // so that counting them (in issue order) tells which slabs have landed
     if constexpr (K::STAGES > 2) wait_vm_c<0>();
-    __syncthreads();
+    slab_barrier();
 #pragma unroll
     for (int s0 = 0; s0 < K::STAGES - 1; ++s0)
       if (s0 < nslab) issue(s0);
@@ -771,7 +780,7 @@ __device__ __forceinline__ void f64_at_body(const GemmBatch &batch) {
         else
           wait_vm_c<(K::STAGES - 2) * K::PER>();
       }
-      __syncthreads();
+      slab_barrier();
       const char *stage = smem + (sidx % K::STAGES) * K::STAGE_BYTES;
       const double *At = reinterpret_cast<const double *>(stage);
       const double *Bs = reinterpret_cast<const double *>(stage + K::A_BYTES);

@@ -66,6 +66,15 @@ __device__ __forceinline__ void wait_vm_c() {
   static_assert(N >= 0 && N < 64, "vmcnt range");
   __builtin_amdgcn_s_waitcnt(0x0F70 | (N & 0xF) | (((N >> 4) & 3) << 14));
 }
+// Workgroup barrier of the slab rings. __syncthreads() brackets s_barrier with
+// workgroup-scope fences, and the release fence makes the compiler wait for every
+// outstanding vector-memory operation (vmcnt(0)) first: that drains the direct-to-LDS
+// loads of the slabs still in flight and leaves a ring one slab deep whatever its number
+// of stages. The rings wait for the loads they need themselves (wait_vm_c) and write no
+// LDS with ordinary stores, so the barrier only has to keep each wave's LDS reads of a
+// slab ahead of the loads that refill its buffer; the memory clobber keeps the compiler
+// from moving memory operations across it.
+__device__ __forceinline__ void slab_barrier() { asm volatile("s_barrier" ::: "memory"); }
 
 // ---------------------------------------------------------------------------
 // slicing
@@ -438,7 +447,7 @@ __device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
 #pragma unroll
       for (int j = 0; j < K::TN; ++j) acc[i][j] = acc_t{};
 
-    __syncthreads();
+    slab_barrier();
 #pragma unroll
     for (int s0 = 0; s0 < STAGES - 1; ++s0)
       if (s0 < nslab) issue(s0);
@@ -453,7 +462,7 @@ __device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
         wait_vm_c<2 * K::PER>();
       else
         wait_vm_c<3 * K::PER>();
-      __syncthreads();
+      slab_barrier();
       if (sidx + STAGES - 1 < nslab) issue(sidx + STAGES - 1);
 
       const char *stage = smem + (sidx % STAGES) * K::STAGE_BYTES;
@@ -872,23 +881,25 @@ __global__ __launch_bounds__(256) void residue_rows_kernel(const double *__restr
   }
 }
 
-// B8 row j (column j of B): residue plane t at bytes [t*kp, t*kp + k). 64 (k) x 64 (j)
-// tiles pass through LDS; each thread then takes four consecutive k of one column.
+// B8 row j (column j of B): residue plane t at bytes [t*kp, t*kp + k). 64 (k) x 32 (j)
+// tiles pass through LDS (17 KiB: registers, not LDS, bound the occupancy); each thread
+// then takes four consecutive k of one column.
 template <int L>
 __global__ __launch_bounds__(256) void residue_cols_kernel(const double *__restrict__ B, long ldb, int k, int n, int p,
                                                            long kp, const int *__restrict__ eB, CrtConsts cc,
                                                            int8_t *__restrict__ B8, long ld8) {
-  __shared__ double tile[64][65];
+  constexpr int TJ = 32;  // columns per tile
+  __shared__ double tile[64][TJ + 1];
   const int k0 = blockIdx.x * 64;
-  const int j0 = blockIdx.y * 64;
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  for (int r = ty; r < 64; r += 4) {
+  const int j0 = blockIdx.y * TJ;
+  const int tx = threadIdx.x % TJ, ty = threadIdx.x / TJ;
+  for (int r = ty; r < 64; r += 256 / TJ) {
     const int kk = k0 + r, jj = j0 + tx;
     tile[r][tx] = (kk < k && jj < n) ? B[(long)kk * ldb + jj] : 0.0;
   }
   __syncthreads();
   const int kq = (threadIdx.x & 15) * 4;  // four consecutive k
-  for (int c = threadIdx.x >> 4; c < 64; c += 16) {
+  for (int c = threadIdx.x >> 4; c < TJ; c += 16) {
     const int jj = j0 + c;
     if (jj >= n) continue;
     const int sh = p - eB[jj];
@@ -1003,7 +1014,7 @@ struct CrtLaunch {
     if (stage == 0) {
       hipLaunchKernelGGL(residue_rows_kernel<L>, dim3(m), dim3(256), 0, s, dA, lda, k, p, kp, eA, cc, A8, ld8);
       PHPC_HIP(hipGetLastError());
-      hipLaunchKernelGGL(residue_cols_kernel<L>, dim3((unsigned)((kp + 63) / 64), (unsigned)((n + 63) / 64)), dim3(256),
+      hipLaunchKernelGGL(residue_cols_kernel<L>, dim3((unsigned)((kp + 63) / 64), (unsigned)((n + 31) / 32)), dim3(256),
                          0, s, dB, ldb, k, n, p, kp, eB, cc, B8, ld8);
       PHPC_HIP(hipGetLastError());
     } else {
