@@ -88,28 +88,48 @@ __device__ __forceinline__ int exp_of_max(double mx) {
   return e;
 }
 
-// One workgroup per row of A: exponent e_i, 2^e_i / mean|a_i.|, non-finite flag.
+// Low-bit code of a finite x: 1075 - q for x = (odd integer) * 2^q, in [104, 2149]; 0 for
+// x = 0. The largest code over a row or column gives its lowest set bit q_min, and with
+// its exponent E (every |x| < 2^E) the width W = E - q_min = E + code - 1075: scaled by
+// 2^(p - E) with p >= W, every entry of that row or column is an integer, exactly.
+__device__ __forceinline__ int low_bit_code(double x) {
+  const uint64_t u = (uint64_t)__double_as_longlong(x) & 0x7FFFFFFFFFFFFFFFull;
+  if (u == 0) return 0;
+  const int be = (int)(u >> 52);
+  const uint64_t man = u & 0xFFFFFFFFFFFFFull;
+  const int q = be ? be - 1075 + __builtin_ctzll(man | (1ull << 52)) : -1074 + __builtin_ctzll(man);
+  return 1075 - q;
+}
+
+// One workgroup per row of A: exponent e_i, 2^e_i / mean|a_i.|, non-finite flag, and the
+// largest exact width over the rows (low_bit_code) into *width.
 __global__ __launch_bounds__(256) void row_stats_kernel(const double *__restrict__ A, long lda, int k,
                                                         int *__restrict__ eA, double *__restrict__ ratioA,
-                                                        int *__restrict__ bad) {
+                                                        int *__restrict__ bad, int *__restrict__ width) {
   __shared__ double red_max[256];
   __shared__ double red_sum[256];
+  __shared__ int red_code[256];
   const int i = blockIdx.x;
   const double *row = A + (long)i * lda;
   double mx = 0.0, sm = 0.0;
+  int code = 0;
   for (int l = threadIdx.x; l < k; l += 256) {
-    const double v = fabs(row[l]);
+    const double x = row[l];
+    const double v = fabs(x);
     if (!(v <= 1.79769313486231570815e308)) *bad = 1;  // inf or nan
     mx = fmax(mx, v);
     sm += v;
+    code = max(code, low_bit_code(x));
   }
   red_max[threadIdx.x] = mx;
   red_sum[threadIdx.x] = sm;
+  red_code[threadIdx.x] = code;
   __syncthreads();
   for (int off = 128; off > 0; off >>= 1) {
     if (threadIdx.x < off) {
       red_max[threadIdx.x] = fmax(red_max[threadIdx.x], red_max[threadIdx.x + off]);
       red_sum[threadIdx.x] += red_sum[threadIdx.x + off];
+      red_code[threadIdx.x] = max(red_code[threadIdx.x], red_code[threadIdx.x + off]);
     }
     __syncthreads();
   }
@@ -117,6 +137,7 @@ __global__ __launch_bounds__(256) void row_stats_kernel(const double *__restrict
     const int e = exp_of_max(red_max[0]);
     eA[i] = e;
     ratioA[i] = red_sum[0] > 0.0 ? ldexp((double)k, e) / red_sum[0] : 0.0;
+    if (red_code[0] > 0) atomicMax(width, e + red_code[0] - 1075);
   }
 }
 
@@ -145,11 +166,13 @@ __global__ void zero_words_kernel(uint32_t *p, long nwords) {
 
 // What the modulus-count rule needs from the row and column ratios (all >= 0 for finite
 // inputs): their maxima, their smallest positive values (+inf when none is positive),
-// and the non-finite flag. max and min are exact, so the result does not depend on the
+// the non-finite flag, and the largest exact widths of A's rows and B's columns (0 when
+// all entries are zero). max and min are exact, so the result does not depend on the
 // reduction order.
 struct RatioSummary {
   double max_a, max_b, minpos_a, minpos_b;
   int bad;
+  int width_a, width_b;
   int pad;
 };
 
@@ -185,7 +208,9 @@ __global__ __launch_bounds__(256) void ratio_summary_kernel(const double *__rest
     out->max_b = red[1][0];
     out->minpos_a = red[2][0];
     out->minpos_b = red[3][0];
-    out->bad = *bad;
+    out->bad = bad[0];
+    out->width_a = bad[1];
+    out->width_b = bad[2];
     out->pad = 0;
   }
 }
@@ -193,31 +218,38 @@ __global__ __launch_bounds__(256) void ratio_summary_kernel(const double *__rest
 // Column maxima / sums of B over a chunk of rows; combined with atomics.
 __global__ __launch_bounds__(256) void col_stats_kernel(const double *__restrict__ B, long ldb, int k, int n,
                                                         int rows_per_block, unsigned long long *__restrict__ cmax,
-                                                        double *__restrict__ csum, int *__restrict__ bad) {
+                                                        double *__restrict__ csum, int *__restrict__ ccode,
+                                                        int *__restrict__ bad) {
   const int j = blockIdx.x * 256 + threadIdx.x;
   if (j >= n) return;
   const int r0 = blockIdx.y * rows_per_block;
   const int r1 = min(k, r0 + rows_per_block);
   double mx = 0.0, sm = 0.0;
+  int code = 0;
   for (int r = r0; r < r1; ++r) {
-    const double v = fabs(B[(long)r * ldb + j]);
+    const double x = B[(long)r * ldb + j];
+    const double v = fabs(x);
     mx = fmax(mx, v);
     sm += v;
+    code = max(code, low_bit_code(x));
   }
   if (!(mx <= 1.79769313486231570815e308)) *bad = 1;
   // non-negative doubles order like their bit patterns
   atomicMax(&cmax[j], (unsigned long long)__double_as_longlong(mx));
   atomicAdd(&csum[j], sm);
+  if (code > 0) atomicMax(&ccode[j], code);
 }
 
-__global__ void col_exps_kernel(const unsigned long long *__restrict__ cmax, const double *__restrict__ csum, int k,
-                                int n, int *__restrict__ eB, double *__restrict__ ratioB) {
+__global__ void col_exps_kernel(const unsigned long long *__restrict__ cmax, const double *__restrict__ csum,
+                                const int *__restrict__ ccode, int k, int n, int *__restrict__ eB,
+                                double *__restrict__ ratioB, int *__restrict__ width) {
   const int j = blockIdx.x * 256 + threadIdx.x;
   if (j >= n) return;
   const double mx = __longlong_as_double((long long)cmax[j]);
   const int e = exp_of_max(mx);
   eB[j] = e;
   ratioB[j] = csum[j] > 0.0 ? ldexp((double)k, e) / csum[j] : 0.0;
+  if (ccode[j] > 0) atomicMax(width, e + ccode[j] - 1075);
 }
 
 // B slices, transposed: B8 row j, slice u at bytes [(s-u)*kp, (s-u)*kp + k), zero to kp.
@@ -1094,7 +1126,7 @@ CrtConsts make_crt(int L) {
 // doubles, 16-byte aligned), column maxima and sums (2 n words), the non-finite flag and
 // the ratio summary.
 size_t split_meta_bytes(int m, int n) {
-  return (size_t)(m + n) * (sizeof(int) + sizeof(double)) + (size_t)n * 16 + 64 + sizeof(RatioSummary) + 16;
+  return (size_t)(m + n) * (sizeof(int) + sizeof(double)) + (size_t)n * 20 + 64 + sizeof(RatioSummary) + 16;
 }
 
 double crt_bits(int L) {
@@ -1193,22 +1225,24 @@ int phpc_dgemm_split_ws(const double *dA, long lda, const double *dB, long ldb, 
   double *ratioB = ratioA + m;
   unsigned long long *cmax = (unsigned long long *)(ratioB + n);
   double *csum = (double *)(cmax + n);
-  int *bad = (int *)(csum + n);
-  RatioSummary *summary = (RatioSummary *)(((uintptr_t)(bad + 1) + 15) & ~(uintptr_t)15);
+  int *bad = (int *)(csum + n);  // [0] non-finite flag, [1] width of A's rows, [2] of B's columns
+  int *ccode = bad + 4;          // per column of B: largest low-bit code
+  RatioSummary *summary = (RatioSummary *)(((uintptr_t)(ccode + n) + 15) & ~(uintptr_t)15);
 
   {
-    const long words = ((long)n * 16 + 16) / 4;  // cmax, csum and the flag
+    const long words = ((long)n * 20 + 16) / 4;  // cmax, csum, the flag and widths, ccode
     hipLaunchKernelGGL(zero_words_kernel, dim3((unsigned)std::min<long>((words + 255) / 256, 4096)), dim3(256), 0, s,
                        (uint32_t *)cmax, words);
     PHPC_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL(row_stats_kernel, dim3(m), dim3(256), 0, s, dA, lda, k, eA, ratioA, bad);
+  hipLaunchKernelGGL(row_stats_kernel, dim3(m), dim3(256), 0, s, dA, lda, k, eA, ratioA, bad, bad + 1);
   PHPC_HIP(hipGetLastError());
   const int rows_per_block = 256;
   hipLaunchKernelGGL(col_stats_kernel, dim3((n + 255) / 256, (k + rows_per_block - 1) / rows_per_block), dim3(256), 0,
-                     s, dB, ldb, k, n, rows_per_block, cmax, csum, bad);
+                     s, dB, ldb, k, n, rows_per_block, cmax, csum, ccode, bad);
   PHPC_HIP(hipGetLastError());
-  hipLaunchKernelGGL(col_exps_kernel, dim3((n + 255) / 256), dim3(256), 0, s, cmax, csum, k, n, eB, ratioB);
+  hipLaunchKernelGGL(col_exps_kernel, dim3((n + 255) / 256), dim3(256), 0, s, cmax, csum, ccode, k, n, eB, ratioB,
+                     bad + 2);
   PHPC_HIP(hipGetLastError());
 
   if (use_crt) {
@@ -1222,6 +1256,7 @@ int phpc_dgemm_split_ws(const double *dA, long lda, const double *dB, long ldb, 
     const double mra = hs.max_a, mrb = hs.max_b;
     const int logk = (int)std::ceil(std::log2((double)std::max(k, 2)));
     int pbits = 0, L = 0;
+    bool exact_scaling = false;
     if (slices > 0) {
       // the caller fixes the modulus count; use the widest integers it can hold, but
       // no wider than residues_of takes: |A'| <= 2^p (|a| < 2^e), and p <= kMaxIntBits
@@ -1235,9 +1270,18 @@ int phpc_dgemm_split_ws(const double *dA, long lda, const double *dB, long ldb, 
                                 std::isinf(hs.minpos_b) ? 1.0 : hs.minpos_b);
       for (pbits = 8; pbits < 70; ++pbits)
         if (crt_error_estimate(pbits, mra, mrb, k) <= target) break;
+      // entries that all fit a narrower width (integers, short mantissas, powers of two)
+      // scale to integers exactly at that width: no truncation error at all, and fewer
+      // bits for the moduli to cover
+      const int wexact = std::max(hs.width_a, hs.width_b);
+      if (wexact >= 1 && wexact < pbits) {
+        pbits = wexact;
+        exact_scaling = true;
+      }
       for (L = 2; L < kMaxMods; ++L)
         if (crt_bits(L) >= crt_needed_bits(pbits, R, k)) break;
       while (pbits > 0 && crt_bits(L) < crt_needed_bits(pbits, R, k)) --pbits;
+      if (pbits < wexact) exact_scaling = false;
     }
     if (pbits < 1) return phpc_set_error(PHPC_ERR_ARG, "k = %d too large for the CRT scheme", k);
     const CrtConsts cc = make_crt(L);
@@ -1284,7 +1328,8 @@ int phpc_dgemm_split_ws(const double *dA, long lda, const double *dB, long ldb, 
       info->slices = L;
       info->products = L;
       info->launches = launches + 7;
-      info->error_estimate = crt_error_estimate(pbits, mra, mrb, k);
+      // exact scaling leaves the two roundings of C += 2^(e_i + f_j - 2p) C'
+      info->error_estimate = exact_scaling ? std::ldexp(1.0, -52) : crt_error_estimate(pbits, mra, mrb, k);
       info->method = 1;
       info->int_bits = pbits;
     }

@@ -188,6 +188,71 @@ def test_crt_auto_moduli_cover_the_largest_product():
     assert np.array_equal(bits(dC.cpu().numpy()), bits(want))
 
 
+def _exact_width(X, axis):
+    """Largest E - q over the rows (axis=1) or columns (axis=0) of X with a nonzero entry:
+    every |x| < 2^E and q is the lowest set bit of any entry, so that 2^(p - E) x is an
+    integer for every entry once p >= E - q."""
+    f, e = np.frexp(X)
+    mant = np.abs(np.ldexp(f, 53)).astype(np.int64)  # odd * 2^(trailing zeros)
+    low = np.where(mant != 0, e - 53 + np.log2(mant & -mant).astype(np.int64), 1 << 20)
+    mx = np.abs(X).max(axis=axis)
+    E = np.frexp(mx)[1]
+    return int((E - low.min(axis=axis))[mx > 0].max())
+
+
+@pytest.mark.parametrize("kind", ["wide", "int"])
+def test_crt_auto_uses_the_exact_width(kind):
+    """Entries that all fit a narrow width below their row's (column's) largest exponent --
+    powers of two over 2^-20 .. 2^20 need 41 bits, integers up to 1000 need 10 -- scale to
+    integers exactly at that width, so the automatic rule takes it in place of the wider
+    integer the error estimate asks for: the result is the restatement's bits at that width
+    and modulus count, and exact (integers) or within two roundings (powers of two) of the
+    double-double product."""
+    import torch
+    m, k, n = 120, 700, 90
+    print(f"crt exact width {kind}", flush=True)
+    if kind == "wide":
+        A, B = O.wide((m, k), 31), O.wide((k, n), 32)
+    else:
+        rng = np.random.default_rng(34)
+        A = rng.integers(-1000, 1001, (m, k)).astype(np.float64)
+        B = rng.integers(-1000, 1001, (k, n)).astype(np.float64)
+    w = max(_exact_width(A, 1), _exact_width(B, 0))
+    assert w == (41 if kind == "wide" else 10)
+    dA, dB = dev(A), dev(B)
+    dC = torch.zeros((m, n), dtype=torch.float64, device="cuda")
+    info = phpc.dgemm_split_device(dA, dB, dC, slices=0, target=1e-14, method=phpc.SPLIT_CRT)
+    torch.cuda.synchronize()
+    print(f"  p={info.int_bits} moduli={info.slices} estimate={info.error_estimate:.2e}", flush=True)
+    assert info.int_bits == w
+    want = O.crt_gemm(A, B, np.zeros((m, n)), info.int_bits, info.slices)
+    got = dC.cpu().numpy()
+    assert np.array_equal(bits(got), bits(want))
+    if kind == "int":
+        assert np.array_equal(got, A @ B)  # |sums| < 2^53: exact in FP64
+    else:
+        rows, cols = np.divmod(np.arange(m * n), n)
+        cw, _ = phpc.check_sampled(dA, dB, dC, rows, cols)
+        assert cw <= 2.0 ** -51
+
+
+def test_crt_auto_full_mantissas_keep_the_estimate():
+    """Uniform entries lie on the 2^-52 grid below 1, an exact width of 52 bits, above the
+    estimate's integer width, which the rule keeps: the smallest p >= 8 with
+    2 * 2^-p (max row ratio + max column ratio) / sqrt(k) <= target."""
+    import torch
+    m, k, n = 100, 900, 80
+    A, B = O.uniform((m, k), 35), O.uniform((k, n), 36)
+    assert max(_exact_width(A, 1), _exact_width(B, 0)) == 52
+    dC = torch.zeros((m, n), dtype=torch.float64, device="cuda")
+    info = phpc.dgemm_split_device(dev(A), dev(B), dC, slices=0, target=1e-14, method=phpc.SPLIT_CRT)
+    torch.cuda.synchronize()
+    ra = (k * np.ldexp(1.0, np.frexp(np.abs(A).max(axis=1))[1]) / np.abs(A).sum(axis=1)).max()
+    rb = (k * np.ldexp(1.0, np.frexp(np.abs(B).max(axis=0))[1]) / np.abs(B).sum(axis=0)).max()
+    p = next(p for p in range(8, 71) if 2 * 2.0 ** -p * (ra + rb) / np.sqrt(k) <= 1e-14)
+    assert info.int_bits == p
+
+
 def test_crt_exact_on_integers():
     """Integer inputs whose product fits the moduli come back exact."""
     import torch
