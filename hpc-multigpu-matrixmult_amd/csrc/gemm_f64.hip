@@ -678,6 +678,10 @@ __device__ __forceinline__ void f64_at_body(const GemmBatch &batch) {
     const int col0 = uni((in_grp / gsz) * BN);
     const int m_rem = uni(d.m - row0);
     const int n_rem = uni(d.n - col0);
+    // single-panel kernels skip the MFMAs of waves with no output rows or columns; the
+    // panel kernels run them (skipping measured slower there: it costs the carried-C loop
+    // more than the few edge tiles give back)
+    const bool wave_live = MP || (wm * K::WTM < m_rem && wn * K::WTN < n_rem);
     const long ldat = uni(d.lda), ldb = uni(d.ldb), ldc = uni(d.ldc);
     const double *At_g = uni(d.A);
     const double *Bg = uni(d.B) + col0;
@@ -798,11 +802,15 @@ __device__ __forceinline__ void f64_at_body(const GemmBatch &batch) {
             acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
       };
       // the first k step's LDS reads go out right after the barrier; the next slab's
-      // loads are issued behind its MFMAs
-      kstep(0);
+      // loads are issued behind its MFMAs. A wave whose rows or columns all lie past the
+      // tile's edge has no output: it only loads its share of the slabs and keeps the
+      // barriers.
+      if (wave_live) kstep(0);
       if (sidx + K::STAGES - 1 < nslab) issue(sidx + K::STAGES - 1);
+      if (wave_live) {
 #pragma unroll
-      for (int kk = 1; kk < K::BK / 4; ++kk) kstep(kk);
+        for (int kk = 1; kk < K::BK / 4; ++kk) kstep(kk);
+      }
       if constexpr (MP) {
         int kb, krem, pe;
         slab_rows(sidx, kb, krem, pe);
