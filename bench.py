@@ -240,6 +240,45 @@ def grouped_bench(phpc, seed=21):
     return out
 
 
+def clock_during(run, seconds=1.5):
+    """Median graphics clock (MHz) that `rocm-smi -g` reports while `run` is called back to
+    back, untimed, for about `seconds`; None when rocm-smi is absent or reports no clock."""
+    import re
+    import threading
+    import torch
+
+    exe = shutil.which("rocm-smi")
+    if exe is None:
+        return None
+    mhz, stop = [], [False]
+
+    def sample():
+        while not stop[0]:
+            try:
+                txt = subprocess.run([exe, "-g", "--json"], capture_output=True, text=True, timeout=10).stdout
+                m = re.search(r"\((\d+)\s*Mhz\)", txt)
+                if m:
+                    mhz.append(int(m.group(1)))
+            except (OSError, subprocess.SubprocessError):
+                pass
+            time.sleep(0.2)
+
+    run()
+    torch.cuda.synchronize()
+    th = threading.Thread(target=sample, daemon=True)
+    th.start()
+    t0 = time.time()
+    while time.time() - t0 < seconds or (len(mhz) < 3 and time.time() - t0 < 4 * seconds):
+        run()
+        torch.cuda.synchronize()
+    stop[0] = True
+    th.join()
+    if not mhz:
+        return None
+    return {"mhz_median": statistics.median(mhz), "mhz_min": min(mhz), "mhz_max": max(mhz), "samples": len(mhz),
+            "source": "rocm-smi -g every 0.2 s while the same work ran back to back, untimed"}
+
+
 def split_sweep(phpc, sizes=(4096, 8192, 16384), seed=31):
     """Split-precision FP64 GEMM at each slice / modulus count: error against the
     double-double product on sampled entries and FP64-equivalent TFLOP/s."""
@@ -463,6 +502,8 @@ def main():
     ap.add_argument("--panel", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-vendor", action="store_true")
+    ap.add_argument("--no-clock", action="store_true",
+                    help="skip sampling the graphics clock (rocm-smi) beside the plain and split GEMMs")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-grouped", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
@@ -597,6 +638,7 @@ def main():
     rows = rng.integers(0, N, 4096)
     cols = rng.integers(0, N, 4096)
     err_cw, err_nw = phpc.check_sampled(A, B, C, rows, cols)
+    plain_clock = clock_during(lambda: run_mode(phpc.SUMMA_CHAINED)) if not args.no_clock else None
 
     detail = {}
     # split-precision FP64 (CRT / Ozaki scheme II on int8 MFMA), same N, both input sets
@@ -636,6 +678,8 @@ def main():
                       "int8_kernel_ms_avg": round(lvl_avg, 4), "int8_launches": lvl_n,
                       "int8_kernel_tops": round(2.0 * N * N * kp / (lvl_avg * 1e-3) / 1e12, 2),
                       "int8_share_of_time": round(lvl_ms / reps / t_ms, 4)}
+        if tag == "uniform" and not args.no_clock:
+            split[tag]["clock"] = clock_during(run_split)
     phpc.fill_device(A, phpc.DIST_UNIFORM, 1)
     phpc.fill_device(B, phpc.DIST_UNIFORM, 2)
     vendor = {}
@@ -711,6 +755,7 @@ def main():
             "pass_ms_avg": round(ms_per_step, 3),
             "timed_unit": "achieved = 2N^3 / kernel_ms_avg (HIP events around the GEMM launch alone, on its "
                           "stream); a step (pass) adds the 4 A row-block transposes",
+            "clock_mhz": (plain_clock or {}).get("mhz_median"),
         },
         "accuracy": {"err_cw_max_sampled": err_cw, "err_nw_sampled": err_nw, "samples": 4096,
                      "reference": "double-double on the device"},
@@ -730,6 +775,11 @@ def main():
             "kernel_ms_avg": uni["int8_kernel_ms_avg"],
             "ops_per_launch": 2.0 * N * N * ((N + 127) // 128 * 128),
             "share_of_split_time": uni["int8_share_of_time"],
+            # the nominal peak assumes 2.4 GHz; under this load the chip holds a lower clock
+            "clock_mhz": (uni.get("clock") or {}).get("mhz_median"),
+            "peak_at_clock": (round(5000.0 * uni["clock"]["mhz_median"] / 2400.0, 1) if uni.get("clock") else None),
+            "frac_at_clock": (round(uni["int8_kernel_tops"] / (5000.0 * uni["clock"]["mhz_median"] / 2400.0), 4)
+                              if uni.get("clock") else None),
         },
         "split": split,
         "summa_panel_exact": {
