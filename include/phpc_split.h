@@ -37,7 +37,10 @@ typedef struct phpc_split_info {
  *                      exact integer product recovered from its residues modulo
  *                      `slices` pairwise coprime moduli <= 256 (one int8 GEMM each) by
  *                      Garner's mixed-radix algorithm; p and the modulus count follow
- *                      from target_err (or p from a fixed modulus count) */
+ *                      from target_err (or p from a fixed modulus count). With the
+ *                      automatic count, inputs whose rows (columns) all fit fewer bits
+ *                      than target_err asks for (integers, short mantissas, powers of
+ *                      two) are scaled at that exact width, with no truncation error */
 #define PHPC_SPLIT_SLICES 0
 #define PHPC_SPLIT_CRT 1
 int phpc_split_set_method(int method);
