@@ -236,6 +236,31 @@ def test_crt_auto_uses_the_exact_width(kind):
         assert cw <= 2.0 ** -51
 
 
+def test_crt_auto_exact_width_with_zero_rows_and_subnormals():
+    """The exact width over rows and columns that are all zero (no width), hold one entry,
+    or hold subnormal entries (lowest set bits down to 2^-1074): the rule takes the largest
+    width of the rest, and the result is the restatement's bits at that width and count."""
+    import torch
+    rng = np.random.default_rng(37)
+    m, k, n = 40, 96, 36
+    A = rng.integers(-50, 51, (m, k)).astype(np.float64) * 0.25
+    B = rng.integers(-50, 51, (k, n)).astype(np.float64)
+    A[3] = 0.0                                        # an all-zero row
+    A[5] = 0.0
+    A[5, 7] = 3.0                                     # a row with one entry
+    A[9] = np.ldexp(rng.integers(-7, 8, k).astype(np.float64), -1070)  # subnormal row
+    B[:, 4] = 0.0                                     # an all-zero column
+    B[:, 6] = np.ldexp(rng.integers(1, 4, k).astype(np.float64), -1072)  # subnormal column
+    w = max(_exact_width(A, 1), _exact_width(B, 0))
+    dC = torch.zeros((m, n), dtype=torch.float64, device="cuda")
+    info = phpc.dgemm_split_device(dev(A), dev(B), dC, slices=0, target=1e-14, method=phpc.SPLIT_CRT)
+    torch.cuda.synchronize()
+    print(f"  width={w} p={info.int_bits} moduli={info.slices}", flush=True)
+    assert info.int_bits == w
+    want = O.crt_gemm(A, B, np.zeros((m, n)), info.int_bits, info.slices)
+    assert np.array_equal(bits(dC.cpu().numpy()), bits(want))
+
+
 def test_crt_auto_full_mantissas_keep_the_estimate():
     """Uniform entries lie on the 2^-52 grid below 1, an exact width of 52 bits, above the
     estimate's integer width, which the rule keeps: the smallest p >= 8 with
