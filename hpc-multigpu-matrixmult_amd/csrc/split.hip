@@ -851,6 +851,18 @@ constexpr int crt_plain_start(int L) {
   return s;
 }
 
+// Lowest digit index F whose Horner value (digits F..L-1, |value| <= prod m_F..m_{L-1} / 2)
+// stays below 2^104: double-double steps there are exact integer arithmetic.
+constexpr int crt_exact_floor(int L) {
+  double prod = 1.0;
+  int f = L;
+  while (f > 0 && prod * kModuli[f - 1] <= 0x1p105) {
+    prod *= kModuli[f - 1];
+    --f;
+  }
+  return f;
+}
+
 // Symmetric residues of one integer-valued double x (|x| < 2^72) for the first L
 // moduli. |x| is cut into nine bytes (three words); for each odd modulus the sum
 // sum_i byte_i (2^(8 i) mod m) <= 9 * 255^2 < 2^20 is two int8 dot products, exact, and
@@ -970,25 +982,39 @@ __device__ __forceinline__ double crt_value(const int (&r)[L], const CrtConsts &
     v[t] = (int)fmaf(-q, (float)cc.m[t], yf);
     vp[t / 4] |= (uint32_t)(v[t] & 0xff) << (8 * (t % 4));
   }
-  // Horner from the top digit: plain FP64 while exact, then double-double
+  // Horner from the top digit: plain FP64 while the prefix stays below 2^53, then
+  // double-double. While the value stays below 2^104 (digits F..L-1) every step is exact
+  // integer arithmetic, so two digits go in a step there (multiplier m_t m_{t-1} < 2^16,
+  // digit v_t m_{t-1} + v_{t-1}, |.| < 2^15) and leave the same (hi, lo) as two steps of
+  // one digit; the digits below F, where only moduli counts above 13 reach, go one at a
+  // time, so the result is the same bits as a one-digit Horner in every case.
   constexpr int S = crt_plain_start(L);
+  constexpr int F = crt_exact_floor(L);
   double hi = (double)v[L - 1], lo = 0.0;
+  int t = L - 2;
 #pragma unroll
-  for (int t = L - 2; t >= S; --t) hi = fma(hi, (double)cc.m[t], (double)v[t]);
-#pragma unroll
-  for (int t = S - 1; t >= 0; --t) {
-    // (hi, lo) = (hi, lo) * m_t + v_t
-    const double mt = (double)cc.m[t];
+  for (; t - 1 >= S; t -= 2)
+    hi = fma(hi, (double)(cc.m[t] * cc.m[t - 1]), (double)(v[t] * cc.m[t - 1] + v[t - 1]));
+  if (t >= S) {
+    hi = fma(hi, (double)cc.m[t], (double)v[t]);
+    --t;
+  }
+  auto dd_step = [&](double mt, double vt) {
+    // (hi, lo) = (hi, lo) * mt + vt
     const double ph = hi * mt;
     const double pe = fma(hi, mt, -ph);
     const double pl = lo * mt + pe;
-    const double sh = ph + (double)v[t];
+    const double sh = ph + vt;
     const double bb = sh - ph;
-    const double se = (ph - (sh - bb)) + ((double)v[t] - bb);
+    const double se = (ph - (sh - bb)) + (vt - bb);
     const double tl = pl + se;
     hi = sh + tl;
     lo = tl - (hi - sh);
-  }
+  };
+#pragma unroll
+  for (; t - 1 >= F; t -= 2) dd_step((double)(cc.m[t] * cc.m[t - 1]), (double)(v[t] * cc.m[t - 1] + v[t - 1]));
+#pragma unroll
+  for (; t >= 0; --t) dd_step((double)cc.m[t], (double)v[t]);
   return hi + lo;
 }
 

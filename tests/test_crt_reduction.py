@@ -7,7 +7,9 @@
 - Garner digits: with the coefficients stored in the symmetric (int8) range, the digit sums
   stay below 2^18, the same rint reduction is exact there, and the mixed-radix digits
   rebuild every integer in the symmetric range of the moduli product; the plain-FP64
-  Horner prefix (top digits while their product is <= 2^53) is exact.
+  Horner prefix (top digits while their product is <= 2^53) is exact;
+- Horner grouping: two digits a step while the value stays below 2^104 leaves the same
+  double-double pair as one digit a step, so the kernel's bits do not depend on it.
 """
 import math
 
@@ -108,3 +110,83 @@ def test_garner_digits_and_horner_rebuild_every_integer():
             for t in range(S - 1, -1, -1):
                 val = val * MODULI[t] + v[t]
             assert val == x, (L, x)
+
+
+def _dd_step(hi, lo, mt, vt):
+    """One double-double Horner step of the reconstruction kernel, (hi, lo) * mt + vt, in
+    IEEE doubles; the fma's exact product error is formed from the integers."""
+    ph = hi * mt
+    pe = float(int(hi) * int(mt) - int(ph))  # fma(hi, mt, -ph): representable, so exact
+    pl = lo * mt + pe
+    sh = ph + vt
+    bb = sh - ph
+    se = (ph - (sh - bb)) + (vt - bb)
+    tl = pl + se
+    h = sh + tl
+    return h, tl - (h - sh)
+
+
+def exact_floor(L):
+    prod, f = 1, L
+    while f > 0 and prod * MODULI[f - 1] <= 2 ** 105:
+        prod *= MODULI[f - 1]
+        f -= 1
+    return f
+
+
+def _horner(v, L, pairs):
+    """The reconstruction kernel's Horner over the mixed-radix digits: plain FP64 while the
+    prefix is at most 2^53, then double-double; with pairs, two digits a step down to
+    exact_floor(L) (one digit a step below it), else one digit a step throughout."""
+    S, F = plain_start(L), exact_floor(L) if pairs else L
+    hi, lo, t = float(v[L - 1]), 0.0, L - 2
+    while pairs and t - 1 >= S:
+        mt, vt = MODULI[t] * MODULI[t - 1], v[t] * MODULI[t - 1] + v[t - 1]
+        assert mt < 2 ** 16 and abs(vt) < 2 ** 15
+        hi = float(int(hi) * mt + vt)  # fma below 2^53: exact
+        t -= 2
+    while t >= S:
+        hi = float(int(hi) * MODULI[t] + v[t])
+        t -= 1
+    assert abs(int(hi)) <= 2 ** 53
+    while t - 1 >= F:
+        mt, vt = MODULI[t] * MODULI[t - 1], v[t] * MODULI[t - 1] + v[t - 1]
+        hi, lo = _dd_step(hi, lo, float(mt), float(vt))
+        t -= 2
+    while t >= 0:
+        hi, lo = _dd_step(hi, lo, float(MODULI[t]), float(v[t]))
+        t -= 1
+    return hi, lo
+
+
+def test_two_digit_horner_steps_match_one_digit_steps():
+    """The reconstruction kernel takes the mixed-radix digits two at a time while the value
+    stays below 2^104 (multiplier m_t m_{t-1} < 2^16, digit v_t m_{t-1} + v_{t-1} with
+    |.| < 2^15). There every step is exact, so the (hi, lo) pair equals the one-digit
+    Horner's (hi = the integer rounded once, lo the rest) and the integer itself; the
+    digits below exact_floor(L), reached by 14 or more moduli, go one at a time, so the
+    result is the one-digit Horner's bits in every case, in and beyond the exact range."""
+    rng = np.random.default_rng(8)
+    for L in (2, 3, 8, 12, 13, 14, 15, 16, 19, 20):
+        invs, q = garner_constants(L)
+        M = math.prod(MODULI[:L])
+        assert math.prod(MODULI[exact_floor(L):L]) <= 2 ** 105
+        for trial in range(300):
+            x = int(rng.integers(-2 ** 62, 2 ** 62)) * int(rng.integers(1, 2 ** 62)) * int(rng.integers(1, 2 ** 40)) % M
+            x = x - M if x >= M // 2 else x
+            if trial == 0:
+                x = M // 2 - 1
+            if trial == 1:
+                x = -(M // 2)
+            if trial % 3 == 2 and M > 2 ** 101:  # values inside the exact range
+                x = x % 2 ** 100 - 2 ** 99
+            r = [sym(x, m) for m in MODULI[:L]]
+            v = [r[0]]
+            for t in range(1, L):
+                y = r[t] * invs[t] + sum(v[s] * q[s][t] for s in range(t))
+                v.append(sym(y, MODULI[t]))
+            hi2, lo2 = _horner(v, L, pairs=True)
+            hi1, lo1 = _horner(v, L, pairs=False)
+            assert (hi2, lo2) == (hi1, lo1), (L, x)
+            if abs(x) < 2 ** 104:
+                assert int(hi2) + int(lo2) == x and hi2 + lo2 == float(x), (L, x)
