@@ -901,6 +901,10 @@ __device__ __forceinline__ uint32_t pack4(int a, int b, int c, int d) {
   return (uint32_t)(a & 0xff) | ((uint32_t)(b & 0xff) << 8) | ((uint32_t)(c & 0xff) << 16) | ((uint32_t)(d & 0xff) << 24);
 }
 
+// residue_cols tile: k rows (a multiple of 4 dividing 128, the plane padding) by columns
+constexpr int kResColsTK = 128, kResColsTJ = 16;
+static_assert(128 % kResColsTK == 0 && kResColsTK == 4 * 32 && 256 % kResColsTJ == 0, "residue_cols tile");
+
 // A8 row i: residue plane t at bytes [t*kp, t*kp + k), zero to kp. Each thread handles
 // four consecutive k and stores one 32-bit word per plane.
 template <int L>
@@ -925,25 +929,26 @@ __global__ __launch_bounds__(256) void residue_rows_kernel(const double *__restr
   }
 }
 
-// B8 row j (column j of B): residue plane t at bytes [t*kp, t*kp + k). 64 (k) x 32 (j)
+// B8 row j (column j of B): residue plane t at bytes [t*kp, t*kp + k). 128 (k) x 16 (j)
 // tiles pass through LDS (17 KiB: registers, not LDS, bound the occupancy); each thread
-// then takes four consecutive k of one column.
+// then takes four consecutive k of two columns, so 32 lanes write one column's 128 bytes
+// of a plane.
 template <int L>
 __global__ __launch_bounds__(256) void residue_cols_kernel(const double *__restrict__ B, long ldb, int k, int n, int p,
                                                            long kp, const int *__restrict__ eB, CrtConsts cc,
                                                            int8_t *__restrict__ B8, long ld8) {
-  constexpr int TJ = 32;  // columns per tile
-  __shared__ double tile[64][TJ + 1];
-  const int k0 = blockIdx.x * 64;
+  constexpr int TK = kResColsTK, TJ = kResColsTJ;
+  __shared__ double tile[TK][TJ + 1];
+  const int k0 = blockIdx.x * TK;
   const int j0 = blockIdx.y * TJ;
   const int tx = threadIdx.x % TJ, ty = threadIdx.x / TJ;
-  for (int r = ty; r < 64; r += 256 / TJ) {
+  for (int r = ty; r < TK; r += 256 / TJ) {
     const int kk = k0 + r, jj = j0 + tx;
     tile[r][tx] = (kk < k && jj < n) ? B[(long)kk * ldb + jj] : 0.0;
   }
   __syncthreads();
-  const int kq = (threadIdx.x & 15) * 4;  // four consecutive k
-  for (int c = threadIdx.x >> 4; c < TJ; c += 16) {
+  const int kq = (threadIdx.x & 31) * 4;  // four consecutive k
+  for (int c = threadIdx.x >> 5; c < TJ; c += 8) {
     const int jj = j0 + c;
     if (jj >= n) continue;
     const int sh = p - eB[jj];
@@ -1072,8 +1077,10 @@ struct CrtLaunch {
     if (stage == 0) {
       hipLaunchKernelGGL(residue_rows_kernel<L>, dim3(m), dim3(256), 0, s, dA, lda, k, p, kp, eA, cc, A8, ld8);
       PHPC_HIP(hipGetLastError());
-      hipLaunchKernelGGL(residue_cols_kernel<L>, dim3((unsigned)((kp + 63) / 64), (unsigned)((n + 31) / 32)), dim3(256),
-                         0, s, dB, ldb, k, n, p, kp, eB, cc, B8, ld8);
+      hipLaunchKernelGGL(residue_cols_kernel<L>,
+                         dim3((unsigned)((kp + kResColsTK - 1) / kResColsTK),
+                              (unsigned)((n + kResColsTJ - 1) / kResColsTJ)),
+                         dim3(256), 0, s, dB, ldb, k, n, p, kp, eB, cc, B8, ld8);
       PHPC_HIP(hipGetLastError());
     } else {
       const long total = (long)m * n;
