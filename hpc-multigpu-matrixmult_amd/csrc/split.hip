@@ -34,11 +34,16 @@ typedef int i4v __attribute__((ext_vector_type(4)));
 typedef int i16v __attribute__((ext_vector_type(16)));
 typedef float f16v __attribute__((ext_vector_type(16)));
 typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+typedef long l2v __attribute__((ext_vector_type(2)));
 typedef __amdgpu_buffer_rsrc_t rsrc_t;
 typedef __attribute__((address_space(3))) void lds_void;
 typedef double __attribute__((address_space(1))) gdouble;
 
 constexpr int kMaxSlices = 12;
+constexpr int kMaxSlicesF8 = 16;  // FP8-E4M3 slices carry 4 bits each
+// FP32 sums of FP8-digit products (|d| <= 15) stay exact integers below 2^24 over at most
+// 74565 products: 225 * 74565 < 2^24
+constexpr long kMaxExactTermsF8 = 74565;
 constexpr long kMaxExactTerms = 131072;  // products per int32 accumulation (127^2 * 2^17 < 2^31)
 // residues reach -128: 128^2 * 130944 < 2^31 (130944 = 2^17 - 128, a whole number of
 // 128-byte slabs)
@@ -141,7 +146,26 @@ __global__ __launch_bounds__(256) void row_stats_kernel(const double *__restrict
   }
 }
 
+// Slice digits: BITS = 7 gives int8 digits |d| <= 127; BITS = 4 gives |d| <= 15, stored as
+// FP8-E4M3 (OCP: sign, 4 exponent bits of bias 7, 3 mantissa bits), which holds every
+// integer up to 16 exactly: d = 1.mmm * 2^q with q = floor(log2 |d|) <= 3.
+__device__ __forceinline__ int8_t e4m3_of(int d) {
+  if (d == 0) return 0;
+  const int a = d < 0 ? -d : d;
+  const int q = 31 - __builtin_clz((unsigned)a);
+  const int mant = ((a << 3) >> q) & 7;
+  return (int8_t)((d < 0 ? 0x80 : 0) | ((q + 7) << 3) | mant);
+}
+template <int BITS>
+__device__ __forceinline__ int8_t slice_digit(double d) {
+  if constexpr (BITS == 4)
+    return e4m3_of((int)d);
+  else
+    return (int8_t)(int)d;
+}
+
 // A8 row i: slice t (1-based) at bytes [(t-1)*kp, (t-1)*kp + k), zero up to kp.
+template <int BITS>
 __global__ __launch_bounds__(256) void slice_rows_kernel(const double *__restrict__ A, long lda, int k, int s,
                                                          long kp, const int *__restrict__ eA,
                                                          int8_t *__restrict__ A8, long ld8) {
@@ -152,10 +176,10 @@ __global__ __launch_bounds__(256) void slice_rows_kernel(const double *__restric
   for (long l = threadIdx.x; l < kp; l += 256) {
     double x = l < k ? ldexp(row[l], -e) : 0.0;
     for (int t = 0; t < s; ++t) {
-      x = ldexp(x, 7);
+      x = ldexp(x, BITS);
       const double d = trunc(x);
       x -= d;
-      out[(long)t * kp + l] = (int8_t)(int)d;
+      out[(long)t * kp + l] = slice_digit<BITS>(d);
     }
   }
 }
@@ -255,6 +279,7 @@ __global__ void col_exps_kernel(const unsigned long long *__restrict__ cmax, con
 // B slices, transposed: B8 row j, slice u at bytes [(s-u)*kp, (s-u)*kp + k), zero to kp.
 // 64 (k) x 64 (j) tiles go through LDS so both the read of B and the write of B8 are
 // coalesced.
+template <int BITS>
 __global__ __launch_bounds__(256) void slice_cols_kernel(const double *__restrict__ B, long ldb, int k, int n, int s,
                                                          long kp, const int *__restrict__ eB, int8_t *__restrict__ B8,
                                                          long ld8) {
@@ -276,10 +301,10 @@ __global__ __launch_bounds__(256) void slice_cols_kernel(const double *__restric
     double x = ldexp(tile[tx][c], -eB[jj]);
     int8_t *out = B8 + (long)jj * ld8;
     for (int u = 1; u <= s; ++u) {
-      x = ldexp(x, 7);
+      x = ldexp(x, BITS);
       const double d = trunc(x);
       x -= d;
-      out[(long)(s - u) * kp + kk] = (int8_t)(int)d;
+      out[(long)(s - u) * kp + kk] = slice_digit<BITS>(d);
     }
   }
 }
@@ -368,8 +393,11 @@ __device__ __forceinline__ int i8_swz(int r) {
 
 // BF = false: int8 operands, v_mfma_i32_32x32x32_i8, int32 sums (level or residue
 // epilogue). BF = true: BF16 operands (8 per 16-byte chunk), v_mfma_f32_32x32x16_bf16,
-// FP32 sums added to C32. The byte layout and the pipeline are the same.
-template <int BM, int BN, int WM, int WN, int BK_, int STAGES, int SD = 16, bool BF = false>
+// FP32 sums added to C32. F8 = true: FP8-E4M3 operands (16 per chunk), two
+// v_mfma_f32_32x32x16_fp8_fp8 per chunk (its low and high eight bytes: A and B take the
+// same bytes, so the sum runs over the same k), FP32 sums that stay exact integers below
+// 2^24, FP64 level epilogue. The byte layout and the pipeline are the same.
+template <int BM, int BN, int WM, int WN, int BK_, int STAGES, int SD = 16, bool BF = false, bool F8 = false>
 __device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
   using K = I8Cfg<BM, BN, WM, WN, BK_, STAGES>;
   extern __shared__ __attribute__((aligned(1024))) char smem[];
@@ -472,7 +500,7 @@ __device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
       }
     };
 
-    using acc_t = typename std::conditional<BF, f16v, i16v>::type;
+    using acc_t = typename std::conditional<BF || F8, f16v, i16v>::type;
     acc_t acc[K::TM][K::TN];
 #pragma unroll
     for (int i = 0; i < K::TM; ++i)
@@ -511,11 +539,16 @@ __device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
         for (int i = 0; i < K::TM; ++i)
 #pragma unroll
           for (int j = 0; j < K::TN; ++j)
-            if constexpr (BF)
+            if constexpr (BF) {
               acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8v, a[i]),
                                                                  __builtin_bit_cast(bf8v, b[j]), acc[i][j], 0, 0, 0);
-            else
+            } else if constexpr (F8) {
+              const l2v av = __builtin_bit_cast(l2v, a[i]), bv = __builtin_bit_cast(l2v, b[j]);
+              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(av.x, bv.x, acc[i][j], 0, 0, 0);
+              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(av.y, bv.y, acc[i][j], 0, 0, 0);
+            } else {
               acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[i], b[j], acc[i][j], 0, 0, 0);
+            }
       }
     }
 
@@ -623,6 +656,13 @@ __global__ __launch_bounds__(1024) void i8_level_256x256_w16_k64_s5(LevelArgs a)
 __global__ __launch_bounds__(1024) void i8_level_256x128_w16_k128_s3(LevelArgs a) {
   i8_level_body<256, 128, 8, 2, 128, 3>(a);
 }
+// FP8-E4M3 digit slices on the same pipeline (FP64-accurate, scheme I with 4-bit digits)
+__global__ __launch_bounds__(1024) void f8_level_256x256_w16_k128(LevelArgs a) {
+  i8_level_body<256, 256, 4, 4, 128, 2, 16, false, true>(a);
+}
+__global__ __launch_bounds__(256) void f8_level_128x128_w4(LevelArgs a) {
+  i8_level_body<128, 128, 2, 2, 64, 4, 16, false, true>(a);
+}
 // BF16 slices on the same pipeline (FP32-accurate 3 x BF16 GEMM)
 __global__ __launch_bounds__(512) void bf16_level_256x256_w8_k128(LevelArgs a) {
   i8_level_body<256, 256, 2, 4, 128, 2, 16, true>(a);
@@ -669,10 +709,18 @@ const I8Config kBfConfigs[] = {
 };
 constexpr int kNumBf = sizeof(kBfConfigs) / sizeof(kBfConfigs[0]);
 
+// FP8-digit level kernels (scheme I with FP8-E4M3 slices)
+const I8Config kF8Configs[] = {
+    I8CFG(256, 256, 4, 4, 128, 2, f8_level_256x256_w16_k128),  // 0: large problems, 16 waves
+    I8CFG(128, 128, 2, 2, 64, 4, f8_level_128x128_w4),         // 1: small problems
+};
+constexpr int kNumF8 = sizeof(kF8Configs) / sizeof(kF8Configs[0]);
+
 std::mutex g_mu;
 bool g_ready = false;
 int g_occ[kNumI8];
 int g_occ_bf[kNumBf];
+int g_occ_f8[kNumF8];
 
 int occupancy_of(const I8Config &c) {
   const void *fn = reinterpret_cast<const void *>(c.fn);
@@ -686,6 +734,7 @@ void init_locked() {
   if (g_ready) return;
   for (int c = 0; c < kNumI8; ++c) g_occ[c] = occupancy_of(kI8Configs[c]);
   for (int c = 0; c < kNumBf; ++c) g_occ_bf[c] = occupancy_of(kBfConfigs[c]);
+  for (int c = 0; c < kNumF8; ++c) g_occ_f8[c] = occupancy_of(kF8Configs[c]);
   g_ready = true;
 }
 
@@ -718,6 +767,16 @@ int launch_level(const LevelArgs &a, int cfg, hipStream_t s) {
     occ = g_occ[cfg];
   }
   return launch_with(kI8Configs[cfg], occ, a, s);
+}
+
+int launch_f8_level(const LevelArgs &a, int cfg, hipStream_t s) {
+  int occ;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    init_locked();
+    occ = g_occ_f8[cfg];
+  }
+  return launch_with(kF8Configs[cfg], occ, a, s);
 }
 
 int launch_bf_level(const LevelArgs &a, int cfg, hipStream_t s) {
@@ -807,8 +866,8 @@ __global__ __launch_bounds__(256) void bf16x3_cols_kernel(const float *__restric
 // Error estimate used to pick the slice count: the truncated terms of one product are
 // below (s+1) 2^{-7s} 2^{e_i+f_j}; they add up like a random walk over k, while
 // (|A||B|)_ij ~ k * mean|a_i.| * mean|b_.j| = k 2^{e_i+f_j} / (ratio_i ratio_j).
-double split_error_estimate(int s, double ratio_a, double ratio_b, int k) {
-  return (s + 1) * std::ldexp(1.0, -7 * s) * ratio_a * ratio_b / std::sqrt((double)std::max(k, 1));
+double split_error_estimate(int s, double ratio_a, double ratio_b, int k, int bits = 7) {
+  return (s + 1) * std::ldexp(1.0, -bits * s) * ratio_a * ratio_b / std::sqrt((double)std::max(k, 1));
 }
 
 // ---------------------------------------------------------------------------
@@ -1190,7 +1249,7 @@ double crt_error_estimate(int p, double ratio_a, double ratio_b, int k) {
   return 2.0 * std::ldexp(1.0, -p) * (ratio_a + ratio_b) / std::sqrt((double)std::max(k, 1));
 }
 
-int method_crt = 1;
+int g_method = PHPC_SPLIT_CRT;
 
 }  // namespace
 
@@ -1203,9 +1262,9 @@ extern "C" int phpc_split_set_config(int cfg) {
 extern "C" int phpc_split_num_configs(void) { return kNumI8; }
 
 extern "C" int phpc_split_set_method(int method) {
-  if (method != PHPC_SPLIT_SLICES && method != PHPC_SPLIT_CRT)
+  if (method != PHPC_SPLIT_SLICES && method != PHPC_SPLIT_CRT && method != PHPC_SPLIT_FP8)
     return phpc_set_error(PHPC_ERR_ARG, "unknown split method %d", method);
-  method_crt = method == PHPC_SPLIT_CRT;
+  g_method = method;
   return phpc_set_error(PHPC_OK, nullptr);
 }
 
@@ -1225,8 +1284,8 @@ extern "C" int phpc_dgemm_split_device(const double *dA, long lda, const double 
   std::lock_guard<std::mutex> lk(g_default_ws.mu);
   hipStream_t s = (hipStream_t)stream_;
   g_default_ws.after_previous(s);
-  const int rc = phpc_dgemm_split_ws(dA, lda, dB, ldb, dC, ldc, m, k, n, slices, target_err,
-                                     method_crt ? PHPC_SPLIT_CRT : PHPC_SPLIT_SLICES, s, &g_default_ws.ws, info);
+  const int rc =
+      phpc_dgemm_split_ws(dA, lda, dB, ldb, dC, ldc, m, k, n, slices, target_err, g_method, s, &g_default_ws.ws, info);
   g_default_ws.mark_used(s);
   return rc;
 }
@@ -1235,13 +1294,18 @@ int phpc_dgemm_split_ws(const double *dA, long lda, const double *dB, long ldb, 
                         int n, int slices, double target_err, int method, hipStream_t stream_, Workspace *ws,
                         phpc_split_info *info) {
   const bool use_crt = method == PHPC_SPLIT_CRT;
+  const bool use_f8 = method == PHPC_SPLIT_FP8;
+  if (method != PHPC_SPLIT_SLICES && !use_crt && !use_f8)
+    return phpc_set_error(PHPC_ERR_ARG, "unknown split method %d", method);
+  const int bits = use_f8 ? 4 : 7;                          // digit width of the slices
+  const int max_slices = use_f8 ? kMaxSlicesF8 : kMaxSlices;
   if (info) *info = phpc_split_info{};
   if (phpc_init()) return phpc_last_error();
   if (m < 0 || k < 0 || n < 0) return phpc_set_error(PHPC_ERR_ARG, "negative size");
   if ((m > 0 && k > 0 && (!dA || lda < k)) || (k > 0 && n > 0 && (!dB || ldb < n)) || (m > 0 && n > 0 && (!dC || ldc < n)))
     return phpc_set_error(PHPC_ERR_ARG, "bad pointers or leading dimensions");
-  if (!use_crt && (slices < 0 || slices > kMaxSlices))
-    return phpc_set_error(PHPC_ERR_ARG, "slices must be 0 (auto) or 1..%d", kMaxSlices);
+  if (!use_crt && (slices < 0 || slices > max_slices))
+    return phpc_set_error(PHPC_ERR_ARG, "slices must be 0 (auto) or 1..%d", max_slices);
   if (use_crt && (slices < 0 || slices == 1 || slices > kMaxMods))
     return phpc_set_error(PHPC_ERR_ARG, "moduli must be 0 (auto) or 2..%d", kMaxMods);
   if (m == 0 || n == 0 || k == 0) return phpc_set_error(PHPC_OK, nullptr);
@@ -1382,34 +1446,39 @@ int phpc_dgemm_split_ws(const double *dA, long lda, const double *dB, long ldb, 
     const double mra = hs.max_a, mrb = hs.max_b;
     if (s_use == 0) {
       const double target = target_err > 0 ? target_err : 1e-14;
-      s_use = kMaxSlices;
-      for (int c = 2; c <= kMaxSlices; ++c)
-        if (split_error_estimate(c, mra, mrb, k) <= target) {
+      s_use = max_slices;
+      for (int c = 2; c <= max_slices; ++c)
+        if (split_error_estimate(c, mra, mrb, k, bits) <= target) {
           s_use = c;
           break;
         }
     }
-    est = split_error_estimate(s_use, mra, mrb, k);
+    est = split_error_estimate(s_use, mra, mrb, k, bits);
   }
   const long ld8a = kp * s_use, ld8b = kp * s_use;
   int8_t *A8 = (int8_t *)ws->get((size_t)m * ld8a, 0, s);
   int8_t *B8 = (int8_t *)ws->get((size_t)n * ld8b, 1, s);
   double *acc = (double *)ws->get((size_t)m * n * 8, 2, s);
   if (!A8 || !B8 || !acc) return phpc_set_error(PHPC_ERR_MEM, "split scratch allocation failed");
-  hipLaunchKernelGGL(slice_rows_kernel, dim3(m), dim3(256), 0, s, dA, lda, k, s_use, kp, eA, A8, ld8a);
+  hipLaunchKernelGGL(use_f8 ? slice_rows_kernel<4> : slice_rows_kernel<7>, dim3(m), dim3(256), 0, s, dA, lda, k, s_use,
+                     kp, eA, A8, ld8a);
   PHPC_HIP(hipGetLastError());
   // B slices in reverse order: slice u at block (s_use - u)
   {
     dim3 grid((unsigned)((kp + 63) / 64), (unsigned)((n + 63) / 64));
-    hipLaunchKernelGGL(slice_cols_kernel, grid, dim3(256), 0, s, dB, ldb, k, n, s_use, kp, eB, B8, ld8b);
+    hipLaunchKernelGGL(use_f8 ? slice_cols_kernel<4> : slice_cols_kernel<7>, grid, dim3(256), 0, s, dB, ldb, k, n,
+                       s_use, kp, eB, B8, ld8b);
     PHPC_HIP(hipGetLastError());
   }
 
   // level GEMMs, smallest level first
   long tiles256 = (long)((m + 255) / 256) * ((n + 255) / 256);
-  const int cfg = tiles256 >= (long)phpc_device().cus / 2 ? g_i8_big : 1;
-  const long piece = kMaxExactTerms / kp * kp;  // whole slices per piece keep offsets aligned
-  const long max_piece = piece > 0 ? piece : (kMaxExactTerms / 64) * 64;
+  const bool big = tiles256 >= (long)phpc_device().cus / 2;
+  const int cfg = use_f8 ? (big ? 0 : 1) : (big ? g_i8_big : 1);
+  // whole slices per piece keep offsets aligned; else whole 128-byte slabs
+  const long max_terms = use_f8 ? kMaxExactTermsF8 : kMaxExactTerms;
+  const long piece = max_terms / kp * kp;
+  const long max_piece = piece > 0 ? piece : (max_terms / 128) * 128;
   bool first = true;
   int launches = 0;
   for (int g = s_use + 1; g >= 2; --g) {
@@ -1426,7 +1495,7 @@ int phpc_dgemm_split_ws(const double *dA, long lda, const double *dB, long ldb, 
       a.m = m;
       a.n = n;
       a.kl = (int)len;
-      a.scale_exp = -7 * g;
+      a.scale_exp = -bits * g;
       a.acc = acc;
       a.ldacc = n;
       a.C = dC;
@@ -1435,7 +1504,7 @@ int phpc_dgemm_split_ws(const double *dA, long lda, const double *dB, long ldb, 
       a.eB = eB;
       a.first = first ? 1 : 0;
       a.last = (g == 2 && off + len >= klen) ? 1 : 0;
-      int rc = launch_level(a, cfg, s);
+      int rc = use_f8 ? launch_f8_level(a, cfg, s) : launch_level(a, cfg, s);
       if (rc) return rc;
       first = false;
       ++launches;
@@ -1446,8 +1515,8 @@ int phpc_dgemm_split_ws(const double *dA, long lda, const double *dB, long ldb, 
     info->products = s_use * (s_use + 1) / 2;
     info->launches = launches + 6;
     info->error_estimate = est;
-    info->method = 0;
-    info->int_bits = 7 * s_use;
+    info->method = method;
+    info->int_bits = bits * s_use;
   }
   return phpc_set_error(PHPC_OK, nullptr);
 }

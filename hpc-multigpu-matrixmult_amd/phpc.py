@@ -357,6 +357,7 @@ def set_tuning(key, value):
 
 SPLIT_SLICES = 0
 SPLIT_CRT = 1
+SPLIT_FP8 = 2
 
 
 def dgemm_split_device(A, B, C, slices=0, target=1e-14, stream=None, method=None):

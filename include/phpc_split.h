@@ -22,12 +22,12 @@ extern "C" {
 #endif
 
 typedef struct phpc_split_info {
-  int slices;            /* slices (PHPC_SPLIT_SLICES) or moduli (PHPC_SPLIT_CRT) used */
+  int slices;            /* slices (PHPC_SPLIT_SLICES, PHPC_SPLIT_FP8) or moduli (PHPC_SPLIT_CRT) used */
   int products;          /* int8 GEMMs computed: s(s+1)/2 slice products, or one per modulus */
   int launches;          /* kernel launches issued */
   double error_estimate; /* a-priori componentwise error estimate used to pick the size */
-  int method;            /* PHPC_SPLIT_SLICES or PHPC_SPLIT_CRT */
-  int int_bits;          /* bits kept per operand entry (7 * slices, or p of the CRT scheme) */
+  int method;            /* PHPC_SPLIT_SLICES, PHPC_SPLIT_CRT or PHPC_SPLIT_FP8 */
+  int int_bits;          /* bits kept per operand entry (7 or 4 * slices, or p of the CRT scheme) */
 } phpc_split_info;
 
 /* Methods:
@@ -40,15 +40,21 @@ typedef struct phpc_split_info {
  *                      from target_err (or p from a fixed modulus count). With the
  *                      automatic count, inputs whose rows (columns) all fit fewer bits
  *                      than target_err asks for (integers, short mantissas, powers of
- *                      two) are scaled at that exact width, with no truncation error */
+ *                      two) are scaled at that exact width, with no truncation error
+ *   PHPC_SPLIT_FP8     Ozaki scheme I with FP8-E4M3 slices: 4-bit digits (|d| <= 15,
+ *                      exact in E4M3), the slice products on the FP8 matrix cores with
+ *                      FP32 accumulation, exact in k pieces of at most 74565 products;
+ *                      about 3x the products of PHPC_SPLIT_SLICES for the same accuracy */
 #define PHPC_SPLIT_SLICES 0
 #define PHPC_SPLIT_CRT 1
+#define PHPC_SPLIT_FP8 2
 int phpc_split_set_method(int method);
 
 /*
- * C[m x n] (ldc) += A[m x k] (lda) * B[k x n] (ldb), FP64, through int8 slices.
- *   slices     : slices 1..12 (PHPC_SPLIT_SLICES) or moduli 2..20 (PHPC_SPLIT_CRT), or 0
- *                to choose the smallest size whose error estimate is at most target_err
+ * C[m x n] (ldc) += A[m x k] (lda) * B[k x n] (ldb), FP64, through int8 (or FP8) slices.
+ *   slices     : slices 1..12 (PHPC_SPLIT_SLICES), 1..16 (PHPC_SPLIT_FP8) or moduli 2..20
+ *                (PHPC_SPLIT_CRT), or 0 to choose the smallest size whose error estimate
+ *                is at most target_err
  *   target_err : componentwise target, max |C - C*| / (|A||B|); <= 0 means 1e-14
  *   info       : optional, filled on success
  * Inputs must be finite (PHPC_ERR_ARG otherwise). The call synchronises the stream once

@@ -66,6 +66,8 @@ def lib():
         h.orc_ramp_closed.argtypes = [I, I, I]
         h.orc_split_gemm.argtypes = [_dp, I, _dp, I, _dp, I, I, I, I, I]
         h.orc_split_digits.argtypes = [ctypes.c_double, I, I, _ip]
+        h.orc_split_gemm_w.argtypes = [_dp, I, _dp, I, _dp, I, I, I, I, I, I, ctypes.c_long]
+        h.orc_split_digits_w.argtypes = [ctypes.c_double, I, I, I, _ip]
         h.orc_exp_of_max.restype = I
         h.orc_exp_of_max.argtypes = [ctypes.c_double]
         h.orc_crt_gemm.argtypes = [_dp, I, _dp, I, _dp, I, I, I, I, I, I]
@@ -230,17 +232,24 @@ def ref_iterative(A, B, C):
     return C
 
 
-def split_gemm(A, B, C, s):
-    """Restatement of the library's split-precision GEMM (C updated in place)."""
+def split_gemm(A, B, C, s, bits=7):
+    """Restatement of the library's split-precision GEMM (C updated in place): 7-bit int8
+    digits, or bits=4 for the FP8-E4M3 slices (k pieces of at most 74565 products)."""
     m, k = A.shape
     n = B.shape[1]
-    lib().orc_split_gemm(_p(A), k, _p(B), n, _p(C), n, m, k, n, s)
+    if bits == 7:
+        lib().orc_split_gemm(_p(A), k, _p(B), n, _p(C), n, m, k, n, s)
+    else:
+        lib().orc_split_gemm_w(_p(A), k, _p(B), n, _p(C), n, m, k, n, s, bits, 74565 if bits == 4 else 131072)
     return C
 
 
-def split_digits(x, e, s):
+def split_digits(x, e, s, bits=7):
     d = (ctypes.c_int * s)()
-    lib().orc_split_digits(float(x), e, s, d)
+    if bits == 7:
+        lib().orc_split_digits(float(x), e, s, d)
+    else:
+        lib().orc_split_digits_w(float(x), e, s, bits, d)
     return list(d)
 
 

@@ -291,20 +291,27 @@ int orc_exp_of_max(double mx) {
   return e;
 }
 
-void orc_split_digits(double x, int e, int s, int *d) {
+void orc_split_digits_w(double x, int e, int s, int bits, int *d) {
   double v = ldexp(x, -e);
   for (int t = 0; t < s; ++t) {
-    v = ldexp(v, 7);
+    v = ldexp(v, bits);
     double q = trunc(v);
     v -= q;
     d[t] = (int)q;
   }
 }
 
+void orc_split_digits(double x, int e, int s, int *d) { orc_split_digits_w(x, e, s, 7, d); }
+
 void orc_split_gemm(const double *A, int lda, const double *B, int ldb, double *C, int ldc, int m, int k, int n,
                     int s) {
+  orc_split_gemm_w(A, lda, B, ldb, C, ldc, m, k, n, s, 7, 131072);
+}
+
+void orc_split_gemm_w(const double *A, int lda, const double *B, int ldb, double *C, int ldc, int m, int k, int n,
+                      int s, int bits, long max_terms) {
   if (m <= 0 || n <= 0 || k <= 0 || s <= 0) return;
-  const long kp = ((long)k + 63) / 64 * 64;
+  const long kp = ((long)k + 127) / 128 * 128; /* the device's plane width */
   int *eA = (int *)malloc(sizeof(int) * (size_t)m);
   int *eB = (int *)malloc(sizeof(int) * (size_t)n);
   signed char *a8 = (signed char *)calloc((size_t)m * (size_t)s * (size_t)k, 1); /* [i][t][l] */
@@ -322,17 +329,18 @@ void orc_split_gemm(const double *A, int lda, const double *B, int ldb, double *
   int d[64];
   for (int i = 0; i < m; ++i)
     for (int l = 0; l < k; ++l) {
-      orc_split_digits(A[(size_t)i * lda + l], eA[i], s, d);
+      orc_split_digits_w(A[(size_t)i * lda + l], eA[i], s, bits, d);
       for (int t = 0; t < s; ++t) a8[((size_t)i * s + t) * k + l] = (signed char)d[t];
     }
   for (int j = 0; j < n; ++j)
     for (int l = 0; l < k; ++l) {
-      orc_split_digits(B[(size_t)l * ldb + j], eB[j], s, d);
+      orc_split_digits_w(B[(size_t)l * ldb + j], eB[j], s, bits, d);
       for (int u = 0; u < s; ++u) b8[((size_t)j * s + u) * k + l] = (signed char)d[u];
     }
-  /* device piece length: whole padded slices per piece, at most 131072 products */
-  long piece = 131072 / kp * kp;
-  if (piece <= 0) piece = (131072 / 64) * 64;
+  /* device piece length: whole padded slices per piece, at most max_terms products, else
+   * whole 128-byte slabs */
+  long piece = max_terms / kp * kp;
+  if (piece <= 0) piece = (max_terms / 128) * 128;
 #pragma omp parallel for schedule(dynamic, 1)
   for (int i = 0; i < m; ++i)
     for (int j = 0; j < n; ++j) {
@@ -352,7 +360,7 @@ void orc_split_gemm(const double *A, int lda, const double *B, int ldb, double *
             const int u = g - t;
             sum += (long long)a8[((size_t)i * s + (t - 1)) * k + l] * (long long)b8[((size_t)j * s + (u - 1)) * k + l];
           }
-          double v = ldexp((double)sum, -7 * g);
+          double v = ldexp((double)sum, -bits * g);
           acc = first ? v : acc + v;
           first = 0;
         }

@@ -126,8 +126,15 @@ double orc_freivalds(const double *A, const double *B, const double *C, int n, u
  * the double-double product within the truncation bound, and error decreasing with s. */
 void orc_split_gemm(const double *A, int lda, const double *B, int ldb, double *C, int ldc, int m, int k, int n,
                     int s);
+/* The same with `bits`-bit digits (x <- 2^bits x) and k pieces of at most `max_terms`
+ * products: bits = 7, max_terms = 131072 is orc_split_gemm; bits = 4, max_terms = 74565
+ * restates the FP8-E4M3 slices (digits |d| <= 15, exact FP32 sums below 2^24). */
+void orc_split_gemm_w(const double *A, int lda, const double *B, int ldb, double *C, int ldc, int m, int k, int n,
+                      int s, int bits, long max_terms);
 /* Digits of one value: d[0..s-1] and exponent e such that x = 2^e sum d_t 2^{-7t} + r. */
 void orc_split_digits(double x, int e, int s, int *d);
+/* The same with `bits`-bit digits: x = 2^e sum d_t 2^{-bits t} + r. */
+void orc_split_digits_w(double x, int e, int s, int bits, int *d);
 int orc_exp_of_max(double mx);
 
 /* ---------------- split-precision, CRT form (Ozaki scheme II) ----------------

@@ -49,6 +49,92 @@ def test_split_k_pieces_match_restatement():
     assert np.array_equal(bits(dC.cpu().numpy()), bits(want))
 
 
+@pytest.mark.parametrize("m,k,n,s,gen", [(1, 1, 1, 1, "u"), (5, 7, 3, 3, "u"), (64, 64, 64, 6, "u"),
+                                         (200, 300, 130, 12, "u"), (257, 129, 255, 9, "w"),
+                                         (96, 1000, 80, 14, "w"), (600, 333, 520, 13, "u")])
+def test_fp8_slices_match_restatement_bitwise(m, k, n, s, gen):
+    """Scheme I on FP8-E4M3 slices (4-bit digits, FP32 sums on the FP8 matrix cores) gives
+    the restatement's bits: the FP32 sums are exact integers, so they equal the oracle's
+    integer level sums, and the FP64 recombination runs in the same order."""
+    import torch
+    print(f"fp8 m={m} k={k} n={n} s={s} {gen}", flush=True)
+    g = O.uniform if gen == "u" else O.wide
+    A, B, C0 = g((m, k), 1), g((k, n), 2), O.uniform((m, n), 3)
+    dC = dev(C0)
+    try:
+        info = phpc.dgemm_split_device(dev(A), dev(B), dC, slices=s, method=phpc.SPLIT_FP8)
+        torch.cuda.synchronize()
+    finally:
+        phpc.lib().phpc_split_set_method(phpc.SPLIT_CRT)
+    assert info.slices == s and info.products == s * (s + 1) // 2 and info.int_bits == 4 * s and info.method == 2
+    want = O.split_gemm(A, B, C0.copy(), s, bits=4)
+    assert np.array_equal(bits(dC.cpu().numpy()), bits(want))
+
+
+def test_fp8_large_problem_kernel_matches_small_one():
+    """3000 x 3000 outputs (144 tiles of 256 x 256) take the large-problem FP8 kernel; the
+    same product in row blocks of 512 takes the small one. Every output's sums and their
+    FP64 order are the same, so the bits are too."""
+    import torch
+    m, k, n, s = 3000, 200, 3000, 5
+    dA = torch.empty((m, k), dtype=torch.float64, device="cuda")
+    dB = torch.empty((k, n), dtype=torch.float64, device="cuda")
+    phpc.fill_device(dA, phpc.DIST_WIDE, 71)
+    phpc.fill_device(dB, phpc.DIST_UNIFORM, 72)
+    whole = torch.zeros((m, n), dtype=torch.float64, device="cuda")
+    blocks = torch.zeros_like(whole)
+    try:
+        phpc.dgemm_split_device(dA, dB, whole, slices=s, method=phpc.SPLIT_FP8)
+        for r0 in range(0, m, 512):
+            r1 = min(m, r0 + 512)
+            phpc.dgemm_split_device(dA[r0:r1], dB, blocks[r0:r1], slices=s, method=phpc.SPLIT_FP8)
+        torch.cuda.synchronize()
+    finally:
+        phpc.lib().phpc_split_set_method(phpc.SPLIT_CRT)
+    assert torch.equal(whole.view(torch.int64), blocks.view(torch.int64))
+
+
+def test_fp8_slices_k_pieces_match_restatement():
+    """k = 20000 with 6 FP8 slices: level 7 spans 6 padded slices (120576 products), more
+    than an exact FP32 sum of 4-bit digit products holds (74565), so it runs in pieces of
+    three slices."""
+    import torch
+    m, k, n, s = 16, 20000, 24, 6
+    A, B = O.uniform((m, k), 4), O.uniform((k, n), 5)
+    dC = torch.zeros((m, n), dtype=torch.float64, device="cuda")
+    try:
+        info = phpc.dgemm_split_device(dev(A), dev(B), dC, slices=s, method=phpc.SPLIT_FP8)
+        torch.cuda.synchronize()
+    finally:
+        phpc.lib().phpc_split_set_method(phpc.SPLIT_CRT)
+    assert info.launches > s + 6  # at least one level was split
+    want = O.split_gemm(A, B, np.zeros((m, n)), s, bits=4)
+    assert np.array_equal(bits(dC.cpu().numpy()), bits(want))
+
+
+@pytest.mark.parametrize("gen", ["u", "w"])
+def test_fp8_slices_auto_reach_fp64_accuracy(gen):
+    """With the count left to the error estimate (4-bit digits), the FP8 slices reach the
+    FP64 target against the double-double product on sampled entries."""
+    import torch
+    N = 2048
+    dA = torch.empty((N, N), dtype=torch.float64, device="cuda")
+    dB = torch.empty_like(dA)
+    dist = phpc.DIST_UNIFORM if gen == "u" else phpc.DIST_WIDE
+    phpc.fill_device(dA, dist, 61)
+    phpc.fill_device(dB, dist, 62)
+    dC = torch.zeros_like(dA)
+    try:
+        info = phpc.dgemm_split_device(dA, dB, dC, slices=0, target=1e-14, method=phpc.SPLIT_FP8)
+        torch.cuda.synchronize()
+    finally:
+        phpc.lib().phpc_split_set_method(phpc.SPLIT_CRT)
+    rng = np.random.default_rng(63)
+    cw, _ = phpc.check_sampled(dA, dB, dC, rng.integers(0, N, 1024), rng.integers(0, N, 1024))
+    print(f"  fp8 auto {gen}: slices={info.slices} estimate={info.error_estimate:.2e} err_cw={cw:.2e}", flush=True)
+    assert info.error_estimate <= 1e-14 and cw <= 1e-14
+
+
 def test_split_exact_on_small_integers():
     import torch
     rng = np.random.default_rng(9)

@@ -278,6 +278,56 @@ def test_split_exact_for_small_integers():
     assert np.array_equal(got, A @ B + C0)
 
 
+def test_split_digits_reconstruct_4_bit():
+    """4-bit digits (the FP8-E4M3 slices): |d| <= 15, x = 2^e sum d_t 2^-4t within
+    2^(e - 4s); exact cases where x fits the digits."""
+    rng = np.random.default_rng(17)
+    for _ in range(200):
+        x = rng.standard_normal() * 2.0 ** rng.integers(-30, 30)
+        e = O.exp_of_max(abs(x))
+        for s in (1, 3, 7, 13):
+            d = O.split_digits(x, e, s, bits=4)
+            assert all(-15 <= t <= 15 for t in d)
+            recon = sum(Fraction(t) * Fraction(2) ** (e - 4 * (i + 1)) for i, t in enumerate(d))
+            assert abs(Fraction(x) - recon) < Fraction(2) ** (e - 4 * s)
+    assert O.split_digits(13.0, 4, 3, bits=4) == [13, 0, 0]
+    assert O.split_digits(-(2.0 ** -5), 0, 3, bits=4) == [0, -8, 0]
+    assert O.split_digits(0.9375 + 0.0625 / 16, 0, 3, bits=4) == [15, 1, 0]
+
+
+def test_split_4_bit_exact_for_small_integers():
+    """Integers up to 15 whose rows and columns reach 8 (exponent 4) are one 4-bit digit:
+    one FP8 slice gives the exact product."""
+    rng = np.random.default_rng(18)
+    A = rng.integers(-15, 16, (9, 40)).astype(np.float64)
+    B = rng.integers(-15, 16, (40, 11)).astype(np.float64)
+    A[:, 0] = 9.0
+    B[0, :] = -12.0
+    C0 = rng.integers(-50, 50, (9, 11)).astype(np.float64)
+    got = O.split_gemm(A, B, C0.copy(), 1, bits=4)
+    assert np.array_equal(got, A @ B + C0)
+
+
+def test_split_4_bit_error_within_truncation_bound_and_decreasing():
+    m, k, n = 24, 300, 20
+    for gen in (O.uniform, O.wide):
+        A, B = gen((m, k), 1), gen((k, n), 2)
+        ref = O.gemm_dd(A, B)
+        eA = np.array([O.exp_of_max(np.abs(A[i]).max()) for i in range(m)])
+        eB = np.array([O.exp_of_max(np.abs(B[:, j]).max()) for j in range(n)])
+        ab = np.abs(A) @ np.abs(B)
+        prev = None
+        for s in (3, 5, 7, 9, 11, 13, 14):
+            C = O.split_gemm(A, B, np.zeros((m, n)), s, bits=4)
+            bound = k * (s + 1) * 2.0 ** (-4 * s) * np.ldexp(1.0, eA[:, None] + eB[None, :]) + 8 * 2.0 ** -53 * ab
+            assert np.all(np.abs(C - ref) <= bound)
+            cw, _ = O.err_full(C, ref, A, B)
+            if prev is not None and prev > 1e-15:
+                assert cw < prev
+            prev = cw
+        assert prev <= 1e-15
+
+
 def test_split_error_within_truncation_bound_and_decreasing():
     m, k, n = 24, 300, 20
     for gen in (O.uniform, O.wide):
