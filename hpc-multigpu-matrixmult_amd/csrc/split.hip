@@ -34,7 +34,7 @@ typedef int i4v __attribute__((ext_vector_type(4)));
 typedef int i16v __attribute__((ext_vector_type(16)));
 typedef float f16v __attribute__((ext_vector_type(16)));
 typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
-typedef long l2v __attribute__((ext_vector_type(2)));
+typedef int i8v __attribute__((ext_vector_type(8)));
 typedef __amdgpu_buffer_rsrc_t rsrc_t;
 typedef __attribute__((address_space(3))) void lds_void;
 typedef double __attribute__((address_space(1))) gdouble;
@@ -393,10 +393,11 @@ __device__ __forceinline__ int i8_swz(int r) {
 
 // BF = false: int8 operands, v_mfma_i32_32x32x32_i8, int32 sums (level or residue
 // epilogue). BF = true: BF16 operands (8 per 16-byte chunk), v_mfma_f32_32x32x16_bf16,
-// FP32 sums added to C32. F8 = true: FP8-E4M3 operands (16 per chunk), two
-// v_mfma_f32_32x32x16_fp8_fp8 per chunk (its low and high eight bytes: A and B take the
-// same bytes, so the sum runs over the same k), FP32 sums that stay exact integers below
-// 2^24, FP64 level epilogue. The byte layout and the pipeline are the same.
+// FP32 sums added to C32. F8 = true: FP8-E4M3 operands (16 per chunk); the chunks of two
+// k steps make one lane's 32 bytes of v_mfma_scale_f32_32x32x64_f8f6f4 at unit scales
+// (A and B take the same bytes, so the sum runs over the same 64 k), FP32 sums that stay
+// exact integers below 2^24, FP64 level epilogue. The byte layout and the pipeline are
+// the same.
 template <int BM, int BN, int WM, int WN, int BK_, int STAGES, int SD = 16, bool BF = false, bool F8 = false>
 __device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
   using K = I8Cfg<BM, BN, WM, WN, BK_, STAGES>;
@@ -526,6 +527,32 @@ __device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
       if (sidx + STAGES - 1 < nslab) issue(sidx + STAGES - 1);
 
       const char *stage = smem + (sidx % STAGES) * K::STAGE_BYTES;
+      if constexpr (F8) {
+        // two k steps' 16-byte chunks make one lane's 32 bytes of a 64-deep FP8 product
+        static_assert(K::KSTEPS % 2 == 0, "FP8 products take k steps in pairs");
+#pragma unroll
+        for (int kk = 0; kk < K::KSTEPS; kk += 2) {
+          i8v a[K::TM], b[K::TN];
+#pragma unroll
+          for (int i = 0; i < K::TM; ++i) {
+            const i4v lo = *reinterpret_cast<const i4v *>(stage + a_row + cofs[kk] + i * 32 * K::BK);
+            const i4v hi = *reinterpret_cast<const i4v *>(stage + a_row + cofs[kk + 1] + i * 32 * K::BK);
+            a[i] = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+          }
+#pragma unroll
+          for (int j = 0; j < K::TN; ++j) {
+            const i4v lo = *reinterpret_cast<const i4v *>(stage + b_row + cofs[kk] + j * 32 * K::BK);
+            const i4v hi = *reinterpret_cast<const i4v *>(stage + b_row + cofs[kk + 1] + j * 32 * K::BK);
+            b[j] = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+          }
+          // formats 0/0: FP8-E4M3 operands; scales 127: 2^0 (E8M0)
+#pragma unroll
+          for (int i = 0; i < K::TM; ++i)
+#pragma unroll
+            for (int j = 0; j < K::TN; ++j)
+              acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a[i], b[j], acc[i][j], 0, 0, 0, 127, 0, 127);
+        }
+      } else {
 #pragma unroll
       for (int kk = 0; kk < K::KSTEPS; ++kk) {
         i4v a[K::TM], b[K::TN];
@@ -542,13 +569,10 @@ __device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
             if constexpr (BF) {
               acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8v, a[i]),
                                                                  __builtin_bit_cast(bf8v, b[j]), acc[i][j], 0, 0, 0);
-            } else if constexpr (F8) {
-              const l2v av = __builtin_bit_cast(l2v, a[i]), bv = __builtin_bit_cast(l2v, b[j]);
-              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(av.x, bv.x, acc[i][j], 0, 0, 0);
-              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(av.y, bv.y, acc[i][j], 0, 0, 0);
-            } else {
+            } else if constexpr (!F8) {
               acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[i], b[j], acc[i][j], 0, 0, 0);
             }
+      }
       }
     }
 
