@@ -280,8 +280,9 @@ def clock_during(run, seconds=1.5):
 
 
 def split_sweep(phpc, sizes=(4096, 8192, 16384), seed=31):
-    """Split-precision FP64 GEMM at each slice / modulus count: error against the
-    double-double product on sampled entries and FP64-equivalent TFLOP/s."""
+    """Split-precision FP64 GEMM at each slice / modulus count (CRT moduli, int8 slices,
+    FP8-E4M3 slices): error against the double-double product on sampled entries and
+    FP64-equivalent TFLOP/s."""
     import numpy as np
     import torch
 
@@ -296,7 +297,8 @@ def split_sweep(phpc, sizes=(4096, 8192, 16384), seed=31):
             phpc.fill_device(A, dist, seed)
             phpc.fill_device(B, dist, seed + 1)
             runs = [(phpc.SPLIT_CRT, 0)] + [(phpc.SPLIT_CRT, L) for L in (8, 10, 12, 14, 16)] + \
-                   [(phpc.SPLIT_SLICES, s_) for s_ in (3, 4, 5, 6, 7, 8)]
+                   [(phpc.SPLIT_SLICES, s_) for s_ in (3, 4, 5, 6, 7, 8)] + \
+                   [(phpc.SPLIT_FP8, s_) for s_ in (0, 3, 4, 5, 6, 7, 8)]
             for method, count in runs:
                 C.zero_()
                 info = phpc.dgemm_split_device(A, B, C, slices=count, target=1e-14, method=method)
@@ -309,9 +311,10 @@ def split_sweep(phpc, sizes=(4096, 8192, 16384), seed=31):
                 e1.record()
                 torch.cuda.synchronize()
                 ms = e0.elapsed_time(e1) / 2
-                out.append({"N": N, "inputs": tag, "method": "crt" if method == phpc.SPLIT_CRT else "slices",
+                out.append({"N": N, "inputs": tag,
+                            "method": {phpc.SPLIT_CRT: "crt", phpc.SPLIT_SLICES: "slices"}.get(method, "fp8"),
                             "requested": count if count else "auto", "count": info.slices,
-                            "int8_gemms": info.products, "int_bits": info.int_bits, "err_cw_max_sampled": cw,
+                            "gemms": info.products, "int_bits": info.int_bits, "err_cw_max_sampled": cw,
                             "error_estimate": info.error_estimate,
                             "tflops_fp64_equiv": round(2.0 * N ** 3 / (ms * 1e-3) / 1e12, 3), "ms": round(ms, 3)})
         del A, B, C
