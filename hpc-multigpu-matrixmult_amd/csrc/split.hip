@@ -554,25 +554,24 @@ __device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
         }
       } else {
 #pragma unroll
-      for (int kk = 0; kk < K::KSTEPS; ++kk) {
-        i4v a[K::TM], b[K::TN];
+        for (int kk = 0; kk < K::KSTEPS; ++kk) {
+          i4v a[K::TM], b[K::TN];
 #pragma unroll
-        for (int i = 0; i < K::TM; ++i)
-          a[i] = *reinterpret_cast<const i4v *>(stage + a_row + cofs[kk] + i * 32 * K::BK);
-#pragma unroll
-        for (int j = 0; j < K::TN; ++j)
-          b[j] = *reinterpret_cast<const i4v *>(stage + b_row + cofs[kk] + j * 32 * K::BK);
-#pragma unroll
-        for (int i = 0; i < K::TM; ++i)
+          for (int i = 0; i < K::TM; ++i)
+            a[i] = *reinterpret_cast<const i4v *>(stage + a_row + cofs[kk] + i * 32 * K::BK);
 #pragma unroll
           for (int j = 0; j < K::TN; ++j)
-            if constexpr (BF) {
-              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8v, a[i]),
-                                                                 __builtin_bit_cast(bf8v, b[j]), acc[i][j], 0, 0, 0);
-            } else if constexpr (!F8) {
-              acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[i], b[j], acc[i][j], 0, 0, 0);
-            }
-      }
+            b[j] = *reinterpret_cast<const i4v *>(stage + b_row + cofs[kk] + j * 32 * K::BK);
+#pragma unroll
+          for (int i = 0; i < K::TM; ++i)
+#pragma unroll
+            for (int j = 0; j < K::TN; ++j)
+              if constexpr (BF)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8v, a[i]),
+                                                                   __builtin_bit_cast(bf8v, b[j]), acc[i][j], 0, 0, 0);
+              else
+                acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
       }
     }
 
