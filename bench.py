@@ -515,6 +515,10 @@ def main():
                     help="run one section alone and print its JSON (for profiler runs)")
     ap.add_argument("--mode", choices=["chained", "fused"], default="chained",
                     help="SUMMA schedule of --only summa")
+    ap.add_argument("--split-method", choices=["crt", "slices", "fp8"], default="crt",
+                    help="method of --only split (CRT moduli, int8 slices or FP8-E4M3 slices)")
+    ap.add_argument("--split-count", type=int, default=0,
+                    help="moduli or slices of --only split (0: chosen for err_cw 1e-14)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy three times (counter calibration)")
     args = ap.parse_args()
@@ -560,7 +564,8 @@ def main():
             res = {"mode": args.mode, "n": args.n, "grid": args.grid, "panel": args.panel,
                    "tflops": round(2.0 * args.n ** 3 / dt / 1e12, 2)}
         elif args.only == "split":
-            # the split CRT GEMM at N (uniform inputs), one warm-up and steps timed calls
+            # the split GEMM at N (uniform inputs), one warm-up and steps timed calls
+            method = {"crt": phpc.SPLIT_CRT, "slices": phpc.SPLIT_SLICES, "fp8": phpc.SPLIT_FP8}[args.split_method]
             A = torch.empty((args.n, args.n), dtype=torch.float64, device="cuda:0")
             B = torch.empty_like(A)
             C = torch.zeros_like(A)
@@ -569,10 +574,11 @@ def main():
             info = None
             t0 = time.perf_counter()
             for _ in range(1 + args.steps):
-                info = phpc.dgemm_split_device(A, B, C, slices=0, target=1e-14, method=phpc.SPLIT_CRT)
+                info = phpc.dgemm_split_device(A, B, C, slices=args.split_count, target=1e-14, method=method)
             torch.cuda.synchronize()
-            res = {"moduli": info.slices, "int_bits": info.int_bits, "calls": 1 + args.steps,
-                   "wall_s": round(time.perf_counter() - t0, 3)}
+            phpc.lib().phpc_split_set_method(phpc.SPLIT_CRT)
+            res = {"method": args.split_method, "count": info.slices, "gemms": info.products,
+                   "int_bits": info.int_bits, "calls": 1 + args.steps, "wall_s": round(time.perf_counter() - t0, 3)}
         elif args.only == "fp32acc":
             res = fp32_accurate_bench(phpc)
         elif args.only == "grouped":
