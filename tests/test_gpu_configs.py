@@ -143,6 +143,33 @@ def test_split_crt_config_sizes(N, dist):
     assert torch.equal(i64(dC), i64(dC2))
 
 
+@pytest.mark.parametrize("dist", [phpc.DIST_UNIFORM, phpc.DIST_WIDE])
+def test_split_fp8_slices_config_size(dist):
+    """The FP8-E4M3 slice rows of the split-count table at N = 8192: the automatic count
+    reaches err_cw 1e-14 on sampled entries against the double-double product, and a
+    second call gives the same bits."""
+    import torch
+    N = 8192
+    print(f"split fp8 N={N} dist={dist}", flush=True)
+    dA = torch.empty((N, N), dtype=torch.float64, device="cuda")
+    dB = torch.empty_like(dA)
+    phpc.fill_device(dA, dist, 1)
+    phpc.fill_device(dB, dist, 2)
+    dC = torch.zeros_like(dA)
+    dC2 = torch.zeros_like(dA)
+    try:
+        info = phpc.dgemm_split_device(dA, dB, dC, slices=0, target=1e-14, method=phpc.SPLIT_FP8)
+        phpc.dgemm_split_device(dA, dB, dC2, slices=0, target=1e-14, method=phpc.SPLIT_FP8)
+        torch.cuda.synchronize()
+    finally:
+        phpc.lib().phpc_split_set_method(phpc.SPLIT_CRT)
+    rows, cols = sample_entries(N, 2048, 5)
+    cw, _ = phpc.check_sampled(dA, dB, dC, rows, cols)
+    print(f"  slices={info.slices} err_cw={cw:.2e} estimate={info.error_estimate:.2e}", flush=True)
+    assert cw <= 1e-14 and info.error_estimate <= 1e-14
+    assert torch.equal(i64(dC), i64(dC2))
+
+
 def f32_sampled_ok(A, B, C, count, seed, C0=None, extra=0.0):
     """C (FP32, on the device) against float64 dot products of the FP32 inputs on `count`
     sampled entries, within ((k + 2) 2^-24 + extra) (|C0| + |A||B|); extra = 2^-23 for the
