@@ -557,19 +557,22 @@ __global__ __launch_bounds__(256) void f32_at_128x128(F32Batch batch) { f32_at_b
 __global__ __launch_bounds__(256) void f32_at_128x128_b4(F32Batch batch) { f32_at_body<true>(batch); }
 
 // At[k][m] = A[m][k] for m < M, k < K; 0 for M <= m < ldat
+// 64 x 64 tiles, 16 entries per thread
+constexpr int kF32TransTile = 64;
 __global__ __launch_bounds__(256) void f32_transpose_kernel(const float *__restrict__ A, long lda, int M, int Kd,
                                                             float *__restrict__ At, long ldat) {
-  __shared__ float tile[32][33];
-  const int k0 = blockIdx.x * 32, m0 = blockIdx.y * 32;
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  constexpr int T = kF32TransTile;
+  __shared__ float tile[T][T + 1];
+  const int k0 = blockIdx.x * T, m0 = blockIdx.y * T;
+  const int tx = threadIdx.x % T, ty = threadIdx.x / T;
 #pragma unroll
-  for (int r = ty; r < 32; r += 8) {
+  for (int r = ty; r < T; r += 256 / T) {
     const int m = m0 + r, k = k0 + tx;
     tile[r][tx] = (m < M && k < Kd) ? A[(long)m * lda + k] : 0.0f;
   }
   __syncthreads();
 #pragma unroll
-  for (int r = ty; r < 32; r += 8) {
+  for (int r = ty; r < T; r += 256 / T) {
     const int k = k0 + r, m = m0 + tx;
     if (k < Kd && m < ldat) At[(long)k * ldat + m] = tile[tx][r];
   }
@@ -686,8 +689,10 @@ int f32_launch_transposed(F32Desc d, hipStream_t stream) {
   g_f32_at_ws.after_previous(stream);
   float *ws = static_cast<float *>(g_f32_at_ws.ws.get(total * sizeof(float), 0, stream));
   if (!ws) return phpc_set_error(PHPC_ERR_MEM, "FP32 staging workspace allocation failed");
-  hipLaunchKernelGGL(f32_transpose_kernel, dim3((unsigned)((d.k + 31) / 32), (unsigned)((ldat + 31) / 32)), dim3(256),
-                     0, stream, d.A, d.lda, d.m, d.k, ws, ldat);
+  hipLaunchKernelGGL(f32_transpose_kernel,
+                     dim3((unsigned)((d.k + kF32TransTile - 1) / kF32TransTile),
+                          (unsigned)((ldat + kF32TransTile - 1) / kF32TransTile)),
+                     dim3(256), 0, stream, d.A, d.lda, d.m, d.k, ws, ldat);
   PHPC_HIP(hipGetLastError());
   d.A = ws;
   d.lda = ldat;
