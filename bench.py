@@ -128,8 +128,9 @@ def traffic_from_profiles(workload):
 def grouped_bench(phpc, seed=21):
     """(a) N = 10000 FP64 on an emulated 3 x 3 grid (rank blocks 3334/3333/3333), panel
     N / lcm = 3333, and (b) 512 independent FP32 GEMMs with M, N, K log-uniform in
-    [64, 4096], each set in one persistent launch replayed from a captured hipGraph, with
-    the tile configuration chosen by the autotuner."""
+    [64, 4096], both as an FP32 plan (FP32 matrix cores) and as an FP32-accurate 3 x BF16
+    plan, each set replayed from a captured hipGraph, with the tile configuration chosen by
+    the autotuner."""
     import numpy as np
     import torch
 
@@ -191,50 +192,54 @@ def grouped_bench(phpc, seed=21):
         phpc.fill_device(b, phpc.DIST_UNIFORM, 10001 + 2 * i)
         gemms.append((a, b, torch.zeros((m, n), dtype=torch.float32, device="cuda")))
         flops += 2.0 * m * n * k
-    plan = phpc.Plan(phpc.F32, gemms, autotune=True, stream=s)
-    for g in gemms:
-        g[2].zero_()
-    torch.cuda.synchronize()
-    with torch.cuda.stream(s):
-        plan.graph_launch(s)
-    s.synchronize()
-    with torch.cuda.stream(s):
-        e0.record(s)
-        for _ in range(reps):
+    for key, dtype in (("fp32_512_gemms", phpc.F32), ("fp32_512_gemms_bf16x3", phpc.F32X3)):
+        plan = phpc.Plan(dtype, gemms, autotune=True, stream=s)
+        for g in gemms:
+            g[2].zero_()
+        torch.cuda.synchronize()
+        with torch.cuda.stream(s):
             plan.graph_launch(s)
-        e1.record(s)
-    s.synchronize()
-    ms = e0.elapsed_time(e1) / reps
-    for g in gemms:
-        g[2].zero_()
-    with torch.cuda.stream(s):
-        plan.graph_launch(s)
-    s.synchronize()
-    # accuracy: 16 GEMMs x 64 entries against float64 dot products of the FP32 inputs
-    worst = 0.0
-    for gi in rng.choice(512, 16, replace=False):
-        a, b, c = gemms[gi]
-        m, k = a.shape
-        n = b.shape[1]
-        r = rng.integers(0, m, 64)
-        cc = rng.integers(0, n, 64)
-        ar = a[r].double().cpu().numpy()
-        bc = b[:, cc].double().cpu().numpy()
-        ref = np.einsum("ik,ki->i", ar, bc)
-        scale = np.einsum("ik,ki->i", np.abs(ar), np.abs(bc))
-        got = c[r, cc].double().cpu().numpy()
-        worst = max(worst, float(np.max(np.abs(got - ref) / np.maximum(scale, 1e-300))))
-    info = plan.info()
-    out["fp32_512_gemms"] = {"tflops": round(flops / (ms * 1e-3) / 1e12, 3), "ms": round(ms, 3),
-                             "flops": flops, "config": info["config"], "tiles": info["tiles"],
-                             "tuned_ms": round(info["tuned_ms"], 3), "err_cw_max_sampled": worst,
-                             "fp32_mfma_util": round(flops / (ms * 1e-3) / 157.3e12, 4),
-                             "launch": ("one batched kernel transposing every A + one persistent "
-                                        "transposed-A kernel for all 512 GEMMs (B read in place)"
-                                        if info["config"] == 3 else
-                                        "one persistent kernel for all 512 GEMMs") +
-                                       ", items in decreasing k, hipGraph replay"}
-    plan.close()
+        s.synchronize()
+        with torch.cuda.stream(s):
+            e0.record(s)
+            for _ in range(reps):
+                plan.graph_launch(s)
+            e1.record(s)
+        s.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        for g in gemms:
+            g[2].zero_()
+        with torch.cuda.stream(s):
+            plan.graph_launch(s)
+        s.synchronize()
+        # accuracy: 16 GEMMs x 64 entries against float64 dot products of the FP32 inputs
+        worst = 0.0
+        for gi in rng.choice(512, 16, replace=False):
+            a, b, c = gemms[gi]
+            m, k = a.shape
+            n = b.shape[1]
+            r = rng.integers(0, m, 64)
+            cc = rng.integers(0, n, 64)
+            ar = a[r].double().cpu().numpy()
+            bc = b[:, cc].double().cpu().numpy()
+            ref = np.einsum("ik,ki->i", ar, bc)
+            scale = np.einsum("ik,ki->i", np.abs(ar), np.abs(bc))
+            got = c[r, cc].double().cpu().numpy()
+            worst = max(worst, float(np.max(np.abs(got - ref) / np.maximum(scale, 1e-300))))
+        info = plan.info()
+        if dtype == phpc.F32X3:
+            launch = ("one batched kernel cutting every A and B into BF16 slice planes + one persistent "
+                      "BF16 level launch (three k segments per item) for all 512 GEMMs"
+                      if info["config"] >= 4 else "one persistent 3 x BF16 kernel for all 512 GEMMs")
+        else:
+            launch = ("one batched kernel transposing every A + one persistent transposed-A kernel for all "
+                      "512 GEMMs (B read in place)" if info["config"] == 3 else
+                      "one persistent kernel for all 512 GEMMs")
+        out[key] = {"tflops": round(flops / (ms * 1e-3) / 1e12, 3), "ms": round(ms, 3), "flops": flops,
+                    "config": info["config"], "tiles": info["tiles"], "tuned_ms": round(info["tuned_ms"], 3),
+                    "err_cw_max_sampled": worst, "fp32_mfma_util": round(flops / (ms * 1e-3) / 157.3e12, 4),
+                    "launch": launch + ", items in decreasing k, hipGraph replay"}
+        plan.close()
     del gemms
     torch.cuda.empty_cache()
     return out
@@ -827,6 +832,9 @@ def main():
             "fp32_512_gemms": {"tflops": g["fp32_512_gemms"]["tflops"],
                                "frac_fp32_peak": g["fp32_512_gemms"]["fp32_mfma_util"],
                                "err_cw": g["fp32_512_gemms"]["err_cw_max_sampled"]},
+            "fp32_512_gemms_bf16x3": {"tflops": g["fp32_512_gemms_bf16x3"]["tflops"],
+                                      "config": g["fp32_512_gemms_bf16x3"]["config"],
+                                      "err_cw": g["fp32_512_gemms_bf16x3"]["err_cw_max_sampled"]},
         }
     if "fp32_accurate" in detail:
         out["fp32_accurate"] = {str(r["N"]): {"fp32_mfma_tflops": r["fp32_mfma"]["tflops"],

@@ -1402,6 +1402,9 @@ extern "C" int phpc_set_tuning(int key, int value) {
       break;
     case 7:
       return phpc_f32_set_plan_config(value);
+    case 8:
+      return phpc_f32x3_set_plan_config(value);
+
     default:
       return phpc_set_error(PHPC_ERR_ARG, "unknown tuning key %d", key);
   }

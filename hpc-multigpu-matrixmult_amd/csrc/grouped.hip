@@ -32,6 +32,9 @@ struct phpc_plan {
   std::vector<F32Desc> d32_at;
   float *at32_ws = nullptr;
   void *copies_dev = nullptr;  // the staging list (F32Staging::list_bytes)
+  // 3 x BF16 plans on slice planes (configurations kF32X3PlanesConfig + g): the planes,
+  // the cut list and the level launch's items
+  Bf3Planes planes;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t graph_stream = nullptr;
@@ -124,8 +127,13 @@ int plan_upload(phpc_plan *p, int id, hipStream_t s) {
     p->vec = true;
     bytes = sizeof(F32Desc) * p->count;
     src = p->d32_at.data();
+  } else if (p->dtype == PHPC_F32X3 && id >= kF32X3PlanesConfig) {
+    if (int rc = p->planes.build(p->d32.data(), p->count, id - kF32X3PlanesConfig)) return rc;
+    p->total_tiles = p->planes.total_tiles;
+    p->config = id;
+    return PHPC_OK;
   } else {
-    // the 3 x BF16 kernel has one tile shape, 128 x 128 (config 2's)
+    // the in-register 3 x BF16 kernel has one tile shape, 128 x 128 (config 2's)
     p->total_tiles = phpc_f32_plan_tiles(p->d32.data(), p->count, p->dtype == PHPC_F32X3 ? 2 : id);
     p->vec = phpc_f32_vec4_ok(p->d32.data(), p->count);
     bytes = sizeof(F32Desc) * p->count;
@@ -150,6 +158,7 @@ int launch(phpc_plan *p, hipStream_t s) {
     if (int rc = p->staging32.run(p->copies_dev, s)) return rc;
     return phpc_f32_launch_at((const F32Desc *)p->descs_dev, p->d32_at.data(), p->count, p->total_tiles, p->grid, s);
   }
+  if (p->dtype == PHPC_F32X3 && p->config >= kF32X3PlanesConfig) return p->planes.run(s, p->grid);
   if (p->dtype == PHPC_F32X3)
     return phpc_f32x3_launch((const F32Desc *)p->descs_dev, p->d32.data(), p->count, p->total_tiles, p->vec, p->grid,
                              s);
@@ -233,7 +242,7 @@ extern "C" int phpc_plan_create(int dtype, const phpc_gemm_item *items, int coun
   const bool at_ok = dtype == PHPC_F64 && phpc_f64_trans_a_ok(p->d64.data(), p->count);
   const bool at32_ok = dtype == PHPC_F32 && phpc_f32_at_ok(p->d32.data(), p->count);
   if (dtype == PHPC_F32) cfg = phpc_f32_pick_plan_config(p->d32.data(), p->count);
-  if (dtype == PHPC_F32X3) cfg = 2;
+  if (dtype == PHPC_F32X3) cfg = phpc_f32x3_pick_plan_config(p->d32.data(), p->count);
   if (flags & PHPC_PLAN_AUTOTUNE) {
     const std::string sig = signature(dtype, live.data(), (int)live.size());
     std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -259,7 +268,9 @@ extern "C" int phpc_plan_create(int dtype, const phpc_gemm_item *items, int coun
         }
         off += (size_t)it.m * it.n * esz;
       }
-      const int ncfg = dtype == PHPC_F64 ? phpc_f64_num_configs() : phpc_f32_num_configs();
+      const int ncfg = dtype == PHPC_F64    ? phpc_f64_num_configs()
+                       : dtype == PHPC_F32X3 ? kF32X3PlanesConfig + phpc_bf3_num_configs()
+                                             : phpc_f32_num_configs();
       double best = 1e30;
       int best_cfg = cfg;
       hipEvent_t e0, e1;
@@ -268,7 +279,7 @@ extern "C" int phpc_plan_create(int dtype, const phpc_gemm_item *items, int coun
       for (int c = 0; c < ncfg; ++c) {
         if (dtype == PHPC_F64 && mp && !phpc_f64_config_panels(c)) continue;
         if (dtype == PHPC_F64 && phpc_f64_config_trans_a(c) && !at_ok) continue;
-        if (dtype == PHPC_F32X3 && c != 2) continue;
+        if (dtype == PHPC_F32X3 && c != 2 && c < kF32X3PlanesConfig) continue;
         if (dtype == PHPC_F32 && c == kF32AtConfig && !at32_ok) continue;
         if (plan_upload(p, c, s) != PHPC_OK) continue;
         p->grid = 0;
@@ -313,6 +324,10 @@ extern "C" int phpc_plan_create(int dtype, const phpc_gemm_item *items, int coun
   if (p->at32_ws && cfg != kF32AtConfig) {
     (void)hipStreamSynchronize(s);
     plan_release_at32(p);
+  }
+  if (p->planes.ws && cfg < kF32X3PlanesConfig) {
+    (void)hipStreamSynchronize(s);
+    p->planes.release();
   }
   p->grid = 0;
   *out = p;
@@ -370,5 +385,6 @@ extern "C" void phpc_plan_destroy(phpc_plan_t p) {
   if (p->descs_dev) (void)hipFree(p->descs_dev);
   if (p->at_ws) (void)hipFree(p->at_ws);
   plan_release_at32(p);
+  p->planes.release();
   delete p;
 }

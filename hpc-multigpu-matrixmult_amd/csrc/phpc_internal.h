@@ -112,6 +112,65 @@ bool phpc_f32_at_ok(const F32Desc *descs, int ndesc);
 int phpc_f32_launch_at(const F32Desc *descs_dev, const F32Desc *descs_host, int ndesc, int total_tiles, int grid,
                        hipStream_t stream);
 bool phpc_f32_vec4_ok(const F32Desc *descs, int ndesc);
+
+// FP32-accurate grouped plans on BF16 slice planes (3 x BF16). One batched kernel cuts
+// every distinct A into rows [hi | mid | lo] and every distinct B into columns
+// [lo | mid | hi] (planes kp = k rounded up to 64 wide, zero past k); one grouped launch
+// of the BF16 level kernel then sums each item's three k segments smallest first into one
+// FP32 accumulator per output and adds it to C: the planes and sums of the single-product
+// slice-plane path (phpc_sgemm_bf16x3_planes), so every item equals that path bit for bit.
+// Plan configurations kF32X3PlanesConfig + g, g < phpc_bf3_num_configs(), select the
+// level kernel's tile.
+constexpr int kF32X3PlanesConfig = 4;
+int phpc_bf3_num_configs();
+// configuration of 3 x BF16 plans created without autotuning: -1 = by work (slice planes on
+// 256 x 256 tiles for at least one such tile per CU; else 2); 2 or kF32X3PlanesConfig + g
+int phpc_f32x3_set_plan_config(int id);
+
+int phpc_f32x3_pick_plan_config(const F32Desc *descs, int ndesc);
+struct Bf3Cut {  // one operand's planes, cut by 64 x 64 tiles of the source
+  const float *src;
+  long ld;
+  int rows, cols;  // source shape: A is m x k, B is k x n
+  uint16_t *dst;   // plane row r (row r of A, column r of B) at dst + r * 3 kp
+  long kp;
+  int is_b;        // B: the source is read across its rows (a transposed cut)
+  int tiles_k;     // 64-wide k tiles across one plane row
+  int block_start; // first workgroup of the batched kernel on this cut
+  int vec;         // A: rows 16-byte aligned (16-byte loads)
+};
+struct Bf3Item {  // one GEMM of the grouped level launch
+  const uint16_t *A3;
+  const uint16_t *B3;
+  float *C;
+  long ldc;
+  int m, n;
+  int kp2;  // bytes of one plane row (2 kp)
+  int tiles_m, tiles_n, tile_start;
+};
+struct Bf3Planes {
+  std::vector<Bf3Cut> cuts;
+  std::vector<size_t> off;      // bytes into the workspace, per cut
+  std::vector<int> block_cut;   // the cut of each workgroup of the batched kernel
+  std::vector<Bf3Item> items;
+  int g = 0;                    // level kernel tile (0 .. phpc_bf3_num_configs() - 1)
+  int blocks = 0, total_tiles = 0;
+  size_t ws_bytes = 0;
+  void *ws = nullptr;   // the planes
+  void *list = nullptr; // cuts, block_cut and items on the device
+  // plans the cuts and tiles of descs for tile g, allocates the planes and uploads the
+  // lists (synchronously); PHPC_ERR_ARG when a plane row is too long for the kernel's
+  // 32-bit offsets
+  int build(const F32Desc *descs, int ndesc, int g);
+  // the cut kernel, then the level kernel; grid 0 = persistent default
+  int run(hipStream_t stream, int grid) const;
+  void release();
+  Bf3Planes() = default;
+  Bf3Planes(const Bf3Planes &) = delete;
+  Bf3Planes &operator=(const Bf3Planes &) = delete;
+  ~Bf3Planes() { release(); }
+};
+
 int phpc_f32x3_launch(const F32Desc *descs_dev, const F32Desc *descs_host, int ndesc, int total_tiles, bool vec4,
                       int grid, hipStream_t stream);
 int phpc_f32_launch(const F32Desc *descs_dev, const F32Desc *descs_host, int ndesc, int total_tiles, int id, bool vec4,

@@ -398,8 +398,39 @@ __device__ __forceinline__ int i8_swz(int r) {
 // (A and B take the same bytes, so the sum runs over the same 64 k), FP32 sums that stay
 // exact integers below 2^24, FP64 level epilogue. The byte layout and the pipeline are
 // the same.
-template <int BM, int BN, int WM, int WN, int BK_, int STAGES, int SD = 16, bool BF = false, bool F8 = false>
-__device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
+// The operands of one level product as the tile code reads them (wave-uniform values):
+// read once per product by the plain walk, once per tile by the grouped one.
+struct LevelOps {
+  const int8_t *A8, *B8;  // at the product's k offset
+  long lda8, ldb8;
+  int m, n, nslab;
+  int sg1, sg2, sb1, sb2;  // BF16 segments
+  float *C32;              // BF16 epilogue
+  long ldc32;
+};
+template <int BK, bool BF>
+__device__ __forceinline__ LevelOps level_ops(const LevelArgs &args) {
+  LevelOps o;
+  o.m = uni(args.m);
+  o.n = uni(args.n);
+  o.nslab = uni(args.kl) / BK;
+  o.lda8 = uni(args.lda8);
+  o.ldb8 = uni(args.ldb8);
+  o.A8 = uni(args.A8) + uni(args.a0);
+  o.B8 = uni(args.B8) + uni(args.b0);
+  o.sg1 = BF ? uni(args.seg1) : 0;
+  o.sg2 = BF ? uni(args.seg2) : 0;
+  o.sb1 = BF ? uni(args.segb1) : 0;
+  o.sb2 = BF ? uni(args.segb2) : 0;
+  o.C32 = BF ? uni(args.C32) : nullptr;
+  o.ldc32 = BF ? uni(args.ldc32) : 0;
+  return o;
+}
+
+// One BM x BN output tile at (row0, col0) of the product ops describes; args carries the
+// int8 and FP8 epilogues' fields (the tile walk is the caller's).
+template <int BM, int BN, int WM, int WN, int BK_, int STAGES, int SD, bool BF, bool F8>
+__device__ __forceinline__ void level_tile(const LevelArgs &args, const LevelOps &ops, int row0, int col0) {
   using K = I8Cfg<BM, BN, WM, WN, BK_, STAGES>;
   extern __shared__ __attribute__((aligned(1024))) char smem[];
   const int tid = threadIdx.x;
@@ -407,19 +438,13 @@ __device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
   const int lane = tid & 63;
   const int wm = wave / WN, wn = wave % WN;
 
-  const int m = uni(args.m), n = uni(args.n), kl = uni(args.kl);
-  const int tiles_n = (n + BN - 1) / BN;
-  const int tiles_m = (m + BM - 1) / BM;
-  const int total = tiles_m * tiles_n;
-  const int G = gridDim.x;
-  const int bid = blockIdx.x;
-  const int slot = uni((G % 8) == 0 ? (bid % 8) * (G / 8) + bid / 8 : bid);
-  const long lda8 = uni(args.lda8), ldb8 = uni(args.ldb8);
-  const int8_t *A8 = uni(args.A8) + uni(args.a0);
-  const int8_t *B8 = uni(args.B8) + uni(args.b0);
-  const int nslab = kl / K::BK;
-  const int sg1 = BF ? uni(args.seg1) : 0, sg2 = BF ? uni(args.seg2) : 0;
-  const int sb1 = BF ? uni(args.segb1) : 0, sb2 = BF ? uni(args.segb2) : 0;
+  const int m = ops.m, n = ops.n;
+  const long lda8 = ops.lda8, ldb8 = ops.ldb8;
+  const int8_t *A8 = ops.A8;
+  const int8_t *B8 = ops.B8;
+  const int nslab = ops.nslab;
+  const int sg1 = ops.sg1, sg2 = ops.sg2;
+  const int sb1 = ops.sb1, sb2 = ops.sb2;
 
   // LDS operand offsets per k step: lane reads 16 bytes of row (l % 32), chunk
   // 2*kk + l/32, at its swizzled slot. The rows a lane reads are l % 32 plus multiples of
@@ -433,17 +458,7 @@ __device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
 #pragma unroll
   for (int kk = 0; kk < K::KSTEPS; ++kk) cofs[kk] = ((2 * kk + (lane >> 5)) ^ sw) << 4;
 
-  for (int round = 0;; ++round) {
-    const int t = round * G + slot;
-    if (t >= total) break;
-    // group 8 tile-rows for L2 reuse
-    const int gm = 8;
-    const int grp = t / (gm * tiles_n);
-    const int first_m = grp * gm;
-    const int gsz = min(tiles_m - first_m, gm);
-    const int in_grp = t - grp * gm * tiles_n;
-    const int row0 = uni((first_m + in_grp % gsz) * BM);
-    const int col0 = uni((in_grp / gsz) * BN);
+  {
     const int m_rem = m - row0, n_rem = n - col0;
     const rsrc_t rA = make_rsrc(A8 + (long)row0 * lda8, (long)min(m_rem, BM) * lda8);
     const rsrc_t rB = make_rsrc(B8 + (long)col0 * ldb8, (long)min(n_rem, BN) * ldb8);
@@ -577,8 +592,8 @@ __device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
 
     // epilogue: 32x32 block, lane l / reg r -> row 8(r/4) + 4(l/32) + r%4, column l%32
     if constexpr (BF) {
-      float __attribute__((address_space(1))) *Cg = (float __attribute__((address_space(1))) *)uni(args.C32);
-      const long ldc32 = uni(args.ldc32);
+      float __attribute__((address_space(1))) *Cg = (float __attribute__((address_space(1))) *)ops.C32;
+      const long ldc32 = ops.ldc32;
 #pragma unroll
       for (int i = 0; i < K::TM; ++i)
 #pragma unroll
@@ -593,7 +608,7 @@ __device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
             *pc = *pc + acc[i][j][r];
           }
         }
-      continue;
+      return;
     }
     const int mod_m = uni(args.mod_m);
     if (mod_m > 0) {
@@ -617,7 +632,7 @@ __device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
             *pr = (int8_t)smod_r(v, mod_m, rmod);
           }
         }
-      continue;
+      return;
     }
     const int sc = uni(args.scale_exp);
     const int first = uni(args.first), last = uni(args.last);
@@ -646,6 +661,31 @@ __device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
           }
         }
       }
+  }
+}
+
+template <int BM, int BN, int WM, int WN, int BK_, int STAGES, int SD = 16, bool BF = false, bool F8 = false>
+__device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
+  const LevelOps ops = level_ops<BK_, BF>(args);
+  const int m = ops.m, n = ops.n;
+  const int tiles_n = (n + BN - 1) / BN;
+  const int tiles_m = (m + BM - 1) / BM;
+  const int total = tiles_m * tiles_n;
+  const int G = gridDim.x;
+  const int bid = blockIdx.x;
+  const int slot = uni((G % 8) == 0 ? (bid % 8) * (G / 8) + bid / 8 : bid);
+  for (int round = 0;; ++round) {
+    const int t = round * G + slot;
+    if (t >= total) break;
+    // group 8 tile-rows for L2 reuse
+    const int gm = 8;
+    const int grp = t / (gm * tiles_n);
+    const int first_m = grp * gm;
+    const int gsz = min(tiles_m - first_m, gm);
+    const int in_grp = t - grp * gm * tiles_n;
+    const int row0 = uni((first_m + in_grp % gsz) * BM);
+    const int col0 = uni((in_grp / gsz) * BN);
+    level_tile<BM, BN, WM, WN, BK_, STAGES, SD, BF, F8>(args, ops, row0, col0);
   }
 }
 
@@ -884,6 +924,171 @@ __global__ __launch_bounds__(256) void bf16x3_cols_kernel(const float *__restric
     *reinterpret_cast<u4v *>(out + kp) = vm;
     *reinterpret_cast<u4v *>(out + 2 * kp) = vh;
   }
+}
+
+// Every cut of a grouped 3 x BF16 plan in one launch: one workgroup per 64 plane rows x 64
+// k of a cut, its cut read from a per-workgroup table. Each thread cuts 8 consecutive k of
+// one plane row (two per thread) and writes 16 bytes per plane. A's rows are its plane
+// rows, so a thread reads its 8 k straight from the row (16-byte loads where rows are
+// aligned); B's tile passes through LDS as tile[plane row][k], read along B's rows
+// (coalesced). The cut is split3, as in bf16x3_rows_kernel / bf16x3_cols_kernel.
+__global__ __launch_bounds__(256) void bf16x3_cut_kernel(const Bf3Cut *__restrict__ cuts,
+                                                         const int *__restrict__ block_cut) {
+  __shared__ float tile[64][65];
+  const int b = blockIdx.x;
+  const Bf3Cut c = cuts[block_cut[b]];
+  const int lb = b - c.block_start;
+  const int r0 = (lb / c.tiles_k) * 64;  // first plane row
+  const int k0 = (lb % c.tiles_k) * 64;
+  const int nrows = c.is_b ? c.cols : c.rows;  // plane rows
+  const int kk = c.is_b ? c.rows : c.cols;     // k
+  if (c.is_b) {
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      // B row k0 + r, columns r0 + tx: plane row tx, k r
+      const int r = ty + 4 * i;
+      const int sk = k0 + r, sj = r0 + tx;
+      tile[tx][r] = (sk < kk && sj < nrows) ? c.src[(long)sk * c.ld + sj] : 0.0f;
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    const int item = threadIdx.x + it * 256;  // 64 plane rows x 8 groups of 8 k
+    const int pr = item >> 3, kq = (item & 7) * 8;
+    if (r0 + pr >= nrows) continue;
+    float x[8];
+    if (c.is_b) {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) x[e] = tile[pr][kq + e];
+    } else {
+      const float *row = c.src + (long)(r0 + pr) * c.ld + k0 + kq;
+      const int krem = kk - (k0 + kq);
+      if (c.vec && krem >= 8) {
+        const float4 v0 = *reinterpret_cast<const float4 *>(row);
+        const float4 v1 = *reinterpret_cast<const float4 *>(row + 4);
+        x[0] = v0.x, x[1] = v0.y, x[2] = v0.z, x[3] = v0.w;
+        x[4] = v1.x, x[5] = v1.y, x[6] = v1.z, x[7] = v1.w;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) x[e] = e < krem ? row[e] : 0.0f;
+      }
+    }
+    uint32_t h[8], mi[8], lo[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) split3(x[e], h[e], mi[e], lo[e]);
+    u4v vh, vm, vl;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      vh[q] = h[2 * q] | (h[2 * q + 1] << 16);
+      vm[q] = mi[2 * q] | (mi[2 * q + 1] << 16);
+      vl[q] = lo[2 * q] | (lo[2 * q + 1] << 16);
+    }
+    uint16_t *out = c.dst + (long)(r0 + pr) * 3 * c.kp + k0 + kq;
+    *reinterpret_cast<u4v *>(out) = c.is_b ? vl : vh;
+    *reinterpret_cast<u4v *>(out + c.kp) = vm;
+    *reinterpret_cast<u4v *>(out + 2 * c.kp) = c.is_b ? vh : vl;
+  }
+}
+
+// One grouped launch of the BF16 level kernel over a 3 x BF16 plan's flat tile list
+// (items in the plan's order, each item's tiles in groups of 8 tile-rows): the persistent
+// workgroups take tiles from XCD-contiguous slots, find the tile's item by binary search
+// on tile_start, and run the level tile on its planes: the three k segments of
+// phpc_sgemm_bf16x3_planes, so the same sum per output.
+struct Bf3Group {
+  const Bf3Item *items;
+  int count;
+  int total_tiles;
+};
+template <int BM, int BN, int WM, int WN, int BK_, int STAGES>
+__device__ __forceinline__ void bf16_group_body(const Bf3Group &g) {
+  const int G = gridDim.x;
+  const int bid = blockIdx.x;
+  const int slot = uni((G % 8) == 0 ? (bid % 8) * (G / 8) + bid / 8 : bid);
+  const int total = uni(g.total_tiles);
+  const int count = uni(g.count);
+  const Bf3Item *items = uni(g.items);
+  const LevelArgs none{};  // the BF16 epilogue reads ops only
+  for (int round = 0;; ++round) {
+    const int t = round * G + slot;
+    if (t >= total) break;
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (uni(items[mid].tile_start) <= t)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const Bf3Item *it = items + lo;
+    const int kp2 = uni(it->kp2);
+    LevelOps o;
+    o.A8 = (const int8_t *)uni(it->A3);
+    o.B8 = (const int8_t *)uni(it->B3);
+    o.lda8 = 3L * kp2;
+    o.ldb8 = 3L * kp2;
+    o.m = uni(it->m);
+    o.n = uni(it->n);
+    o.nslab = 6 * kp2 / BK_;
+    o.sg1 = 3 * kp2;
+    o.sg2 = 5 * kp2;
+    o.sb1 = kp2;
+    o.sb2 = 2 * kp2;
+    o.C32 = uni(it->C);
+    o.ldc32 = uni(it->ldc);
+    const int lt = t - uni(it->tile_start);
+    const int tiles_m = uni(it->tiles_m), tiles_n = uni(it->tiles_n);
+    const int gm = 8;
+    const int grp = lt / (gm * tiles_n);
+    const int first_m = grp * gm;
+    const int gsz = min(tiles_m - first_m, gm);
+    const int in_grp = lt - grp * gm * tiles_n;
+    const int row0 = uni((first_m + in_grp % gsz) * BM);
+    const int col0 = uni((in_grp / gsz) * BN);
+    level_tile<BM, BN, WM, WN, BK_, STAGES, 16, true, false>(none, o, row0, col0);
+  }
+}
+__global__ __launch_bounds__(256) void bf16_group_128x128_w4(Bf3Group g) { bf16_group_body<128, 128, 2, 2, 64, 4>(g); }
+__global__ __launch_bounds__(512) void bf16_group_256x128_w8_k128(Bf3Group g) {
+  bf16_group_body<256, 128, 4, 2, 128, 2>(g);
+}
+__global__ __launch_bounds__(1024) void bf16_group_256x256_w16_k128(Bf3Group g) {
+  bf16_group_body<256, 256, 4, 4, 128, 2>(g);
+}
+
+struct Bf3GroupConfig {
+  int bm, bn, bk, threads;
+  size_t lds;
+  void (*fn)(Bf3Group);
+};
+#define BF3CFG(BM_, BN_, WM_, WN_, BK_, ST_, FN_) \
+  Bf3GroupConfig { BM_, BN_, BK_, WM_ *WN_ * 64, (size_t)(ST_)*I8Cfg<BM_, BN_, WM_, WN_, BK_, ST_>::STAGE_BYTES, FN_ }
+const Bf3GroupConfig kBf3Group[] = {
+    BF3CFG(128, 128, 2, 2, 64, 4, bf16_group_128x128_w4),         // 0: 4 waves, 64x64 wave tiles
+    BF3CFG(256, 128, 4, 2, 128, 2, bf16_group_256x128_w8_k128),   // 1: 8 waves, 64x64 wave tiles
+    BF3CFG(256, 256, 4, 4, 128, 2, bf16_group_256x256_w16_k128),  // 2: 16 waves, 64x64 wave tiles
+};
+constexpr int kNumBf3Group = sizeof(kBf3Group) / sizeof(kBf3Group[0]);
+int g_occ_bf3[kNumBf3Group];
+bool g_bf3_ready = false;
+
+int bf3_occupancy(int g) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_bf3_ready) {
+    for (int c = 0; c < kNumBf3Group; ++c) {
+      const void *fn = reinterpret_cast<const void *>(kBf3Group[c].fn);
+      (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kBf3Group[c].lds);
+      int n = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, kBf3Group[c].threads, kBf3Group[c].lds) != hipSuccess ||
+          n < 1)
+        n = 1;
+      g_occ_bf3[c] = n;
+    }
+    g_bf3_ready = true;
+  }
+  return g_occ_bf3[g];
 }
 
 // Error estimate used to pick the slice count: the truncated terms of one product are
@@ -1584,6 +1789,145 @@ int phpc_sgemm_bf16x3_planes(const float *dA, long lda, const float *dB, long ld
   a.C32 = dC;
   a.ldc32 = ldc;
   return launch_bf_level(a, cfg, s);
+}
+
+// ---------------------------------------------------------------------------
+// Grouped 3 x BF16 plans on slice planes (declared in phpc_internal.h)
+// ---------------------------------------------------------------------------
+int phpc_bf3_num_configs() { return kNumBf3Group; }
+
+namespace {
+size_t bf3_align(size_t b) { return (b + 255) / 256 * 256; }
+int g_f32x3_plan_cfg = -1;
+}  // namespace
+
+int phpc_f32x3_set_plan_config(int id) {
+  if (id != -1 && id != 2 && (id < kF32X3PlanesConfig || id >= kF32X3PlanesConfig + kNumBf3Group))
+    return phpc_set_error(PHPC_ERR_ARG, "3 x BF16 plan config %d out of range", id);
+  g_f32x3_plan_cfg = id;
+  return phpc_set_error(PHPC_OK, nullptr);
+}
+
+int phpc_f32x3_pick_plan_config(const F32Desc *descs, int ndesc) {
+  if (g_f32x3_plan_cfg >= 0) return g_f32x3_plan_cfg;
+  // slice planes on 256 x 256 tiles (the fastest tile on the 512-GEMM mix) once the items
+  // give every CU at least one such tile
+  long work = 0;
+  for (int i = 0; i < ndesc; ++i) work += (long)((descs[i].m + 255) / 256) * ((descs[i].n + 255) / 256);
+  return work >= (long)phpc_device().cus ? kF32X3PlanesConfig + 2 : 2;
+}
+
+int Bf3Planes::build(const F32Desc *descs, int ndesc, int gcfg) {
+  release();
+  cuts.clear();
+  off.clear();
+  block_cut.clear();
+  items.clear();
+  if (gcfg < 0 || gcfg >= kNumBf3Group) return phpc_set_error(PHPC_ERR_ARG, "slice-plane tile %d out of range", gcfg);
+  g = gcfg;
+  const Bf3GroupConfig &cf = kBf3Group[g];
+  blocks = 0;
+  total_tiles = 0;
+  ws_bytes = 0;
+  // one cut per distinct operand (an A or B shared by several items is cut once)
+  auto find_or_add = [&](const float *src, long ld, int rows, int cols, int is_b, long kp) {
+    for (size_t j = 0; j < cuts.size(); ++j) {
+      const Bf3Cut &c = cuts[j];
+      if (c.src == src && c.ld == ld && c.rows == rows && c.cols == cols && c.is_b == is_b) return (int)j;
+    }
+    Bf3Cut c{};
+    c.src = src;
+    c.ld = ld;
+    c.rows = rows;
+    c.cols = cols;
+    c.kp = kp;
+    c.is_b = is_b;
+    c.vec = !is_b && (reinterpret_cast<uintptr_t>(src) & 15) == 0 && ld % 4 == 0;
+    c.tiles_k = (int)(kp / 64);
+    c.block_start = blocks;
+    const int prow = is_b ? cols : rows;
+    const int nt = c.tiles_k * ((prow + 63) / 64);
+    blocks += nt;
+    block_cut.insert(block_cut.end(), nt, (int)cuts.size());
+    cuts.push_back(c);
+    off.push_back(ws_bytes);
+    ws_bytes += bf3_align((size_t)prow * 3 * kp * 2);
+    return (int)cuts.size() - 1;
+  };
+  std::vector<int> ca, cb;
+  for (int i = 0; i < ndesc; ++i) {
+    const F32Desc &d = descs[i];
+    if (d.m <= 0 || d.n <= 0 || d.k <= 0) continue;
+    const long kp = ((long)d.k + 63) / 64 * 64;
+    // a tile's plane rows are read through 32-bit buffer offsets, and so is the k offset
+    if ((long)std::max(cf.bm, cf.bn) * 3 * kp * 2 >= 0x7fffffffL)
+      return phpc_set_error(PHPC_ERR_ARG, "k = %d too large for the slice planes", d.k);
+    ca.push_back(find_or_add(d.A, d.lda, d.m, d.k, 0, kp));
+    cb.push_back(find_or_add(d.B, d.ldb, d.k, d.n, 1, kp));
+    Bf3Item it{};
+    it.C = d.C;
+    it.ldc = d.ldc;
+    it.m = d.m;
+    it.n = d.n;
+    it.kp2 = (int)(kp * 2);
+    it.tiles_m = (d.m + cf.bm - 1) / cf.bm;
+    it.tiles_n = (d.n + cf.bn - 1) / cf.bn;
+    it.tile_start = total_tiles;
+    total_tiles += it.tiles_m * it.tiles_n;
+    items.push_back(it);
+  }
+  if (items.empty()) return PHPC_OK;
+  if (hipMalloc(&ws, ws_bytes) != hipSuccess) {
+    ws = nullptr;
+    return phpc_set_error(PHPC_ERR_MEM, "slice-plane workspace allocation failed (%zu bytes)", ws_bytes);
+  }
+  char *base = static_cast<char *>(ws);
+  for (size_t j = 0; j < cuts.size(); ++j) cuts[j].dst = reinterpret_cast<uint16_t *>(base + off[j]);
+  for (size_t i = 0; i < items.size(); ++i) {
+    items[i].A3 = cuts[ca[i]].dst;
+    items[i].B3 = cuts[cb[i]].dst;
+  }
+  const size_t cuts_b = bf3_align(cuts.size() * sizeof(Bf3Cut));
+  const size_t bc_b = bf3_align(block_cut.size() * sizeof(int));
+  const size_t it_b = items.size() * sizeof(Bf3Item);
+  if (hipMalloc(&list, cuts_b + bc_b + it_b) != hipSuccess) {
+    list = nullptr;
+    release();
+    return phpc_set_error(PHPC_ERR_MEM, "slice-plane list allocation failed");
+  }
+  char *l = static_cast<char *>(list);
+  PHPC_HIP(hipMemcpy(l, cuts.data(), cuts.size() * sizeof(Bf3Cut), hipMemcpyHostToDevice));
+  PHPC_HIP(hipMemcpy(l + cuts_b, block_cut.data(), block_cut.size() * sizeof(int), hipMemcpyHostToDevice));
+  PHPC_HIP(hipMemcpy(l + cuts_b + bc_b, items.data(), it_b, hipMemcpyHostToDevice));
+  return PHPC_OK;
+}
+
+int Bf3Planes::run(hipStream_t s, int grid) const {
+  if (items.empty() || total_tiles <= 0) return PHPC_OK;
+  const char *l = static_cast<const char *>(list);
+  const size_t cuts_b = bf3_align(cuts.size() * sizeof(Bf3Cut));
+  const size_t bc_b = bf3_align(block_cut.size() * sizeof(int));
+  hipLaunchKernelGGL(bf16x3_cut_kernel, dim3((unsigned)blocks), dim3(256), 0, s, reinterpret_cast<const Bf3Cut *>(l),
+                     reinterpret_cast<const int *>(l + cuts_b));
+  PHPC_HIP(hipGetLastError());
+  const Bf3GroupConfig &cf = kBf3Group[g];
+  const long G = std::min<long>(grid > 0 ? grid : (long)bf3_occupancy(g) * phpc_device().cus, total_tiles);
+  Bf3Group a{};
+  a.items = reinterpret_cast<const Bf3Item *>(l + cuts_b + bc_b);
+  a.count = (int)items.size();
+  a.total_tiles = total_tiles;
+  const int kt = phpc_ktime_begin(s);
+  hipLaunchKernelGGL(cf.fn, dim3((unsigned)G), dim3(cf.threads), cf.lds, s, a);
+  phpc_ktime_end(kt, s);
+  PHPC_HIP(hipGetLastError());
+  return PHPC_OK;
+}
+
+void Bf3Planes::release() {
+  if (list) (void)hipFree(list);
+  if (ws) (void)hipFree(ws);
+  list = nullptr;
+  ws = nullptr;
 }
 
 // ---------------------------------------------------------------------------

@@ -96,7 +96,11 @@ int phpc_num_configs(void);
  * autotuning off configuration 3; key 7 = FP32 configuration of plans created
  * without autotuning (-1 = by work, default: 3 for at least two 128x128 tiles per CU, else
  * 0; 0-2 = register-staged 64x64, 128x64, 128x128; 3 = transposed-A 128x128, with the plan
- * keeping staged operand copies). */
+ * keeping staged operand copies); key 8 = configuration of 3 x BF16 (PHPC_F32X3) plans
+ * created without autotuning (-1 = by work, default: 6 for at least one 256x256 tile per
+ * CU, else 2; 2 = BF16 slices cut in registers, 128x128 tiles; 4, 5, 6 = BF16 slice planes
+ * cut once per operand, then one grouped level launch on 128x128, 256x128 or 256x256 tiles,
+ * the plan keeping the planes; configurations 4, 5 and 6 give the same bits). */
 int phpc_set_tuning(int key, int value);
 /* Describe configuration `id`: macro tile bm x bn, threads per workgroup. */
 int phpc_config_info(int id, int *bm, int *bn, int *threads);

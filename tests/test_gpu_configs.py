@@ -8,7 +8,8 @@
 - the split-precision GEMM at N = 8192 and 16384 on both input sets, sampled against the
   double-double product;
 - 512 FP32 GEMMs with M, N, K log-uniform in [64, 4096] in one plan, every GEMM within
-  the FP32 bound, hipGraph replays equal to the eager launch;
+  the FP32 bound, hipGraph replays equal to the eager launch; the same GEMMs as a 3 x BF16
+  plan on slice planes within the 3 x BF16 bound;
 - a Poisson serving stream with the 60/30/10 size law up to 8192, FP64/FP32 classes 50/50
   and a 200 GB budget: every result equals the same request through the one-shot entry
   points bit for bit and meets its class's bound.
@@ -232,6 +233,35 @@ def test_grouped_fp32_512_gemms_config_law():
         _ = torch.ones(1024, device="cuda").sum().item()  # eager work between replays
         for i, g in enumerate(gemms):
             assert torch.equal(g[2].view(torch.int32), eager[i].view(torch.int32)), (rep, i)
+    plan.close()
+
+
+def test_grouped_fp32_512_gemms_bf16x3_slice_planes():
+    """The same 512 FP32 GEMMs as an FP32-accurate 3 x BF16 plan on slice planes (one cut
+    of every operand, one grouped BF16 level launch): every GEMM within the 3 x BF16 bound
+    ((k + 2) 2^-24 + 2^-23) (|A||B|) on sampled entries."""
+    import torch
+    rng = np.random.default_rng(21)
+    shapes = np.rint(np.exp(rng.uniform(np.log(64), np.log(4096), size=(512, 3)))).astype(int)
+    gemms = []
+    for i, (m, n, k) in enumerate(shapes):
+        a = torch.empty((m, k), dtype=torch.float32, device="cuda")
+        b = torch.empty((k, n), dtype=torch.float32, device="cuda")
+        phpc.fill_device(a, phpc.DIST_UNIFORM, 10000 + 2 * i)
+        phpc.fill_device(b, phpc.DIST_UNIFORM, 10001 + 2 * i)
+        gemms.append((a, b, torch.zeros((m, n), dtype=torch.float32, device="cuda")))
+    plan = phpc.Plan(phpc.F32X3, gemms)
+    info = plan.info()
+    print(f"  plan {info}", flush=True)
+    assert info["config"] == 6  # slice planes on 256 x 256 tiles, by work
+    plan.launch()
+    torch.cuda.synchronize()
+    worst = 0.0
+    for i, (a, b, c) in enumerate(gemms):
+        ok, rel = f32_sampled_ok(a, b, c, 32, i, extra=2.0 ** -23)
+        worst = max(worst, rel)
+        assert ok, (i, tuple(shapes[i]))
+    print(f"  512 GEMMs in bound, worst err_cw {worst:.2e}", flush=True)
     plan.close()
 
 

@@ -154,7 +154,10 @@ def test_bf16x3_plan_graph_replays_equal_eager():
         host.append((A, B))
     s = torch.cuda.Stream()
     pg = phpc.Plan(phpc.F32X3, gem_g, autotune=True, stream=s)
-    pe = phpc.Plan(phpc.F32X3, gem_e)
+    # the same shape list: the tuned configuration comes from the cache, so both plans
+    # run the same kernels
+    pe = phpc.Plan(phpc.F32X3, gem_e, autotune=True)
+    assert pg.info()["config"] == pe.info()["config"]
     junk = torch.randn(512, 512, device="cuda")
     for rep in range(3):
         for g in gem_g + gem_e:
@@ -204,3 +207,104 @@ def test_bf16x3_slice_planes_exact_on_small_integers():
     torch.cuda.synchronize()
     want = (A.astype(np.int64) @ B.astype(np.int64)).astype(np.float32)
     assert np.array_equal(dC.cpu().numpy(), want)
+
+
+def _x3_plan(items, cfg, stream=None):
+    """A 3 x BF16 plan pinned to configuration cfg (tuning key 8)."""
+    phpc.set_tuning(8, cfg)
+    try:
+        plan = phpc.Plan(phpc.F32X3, items, stream=stream)
+    finally:
+        phpc.set_tuning(8, -1)
+    assert plan.info()["config"] == cfg
+    return plan
+
+
+@pytest.mark.parametrize("cfg", [4, 5, 6])
+def test_bf16x3_plan_slice_planes_equal_single_path(cfg):
+    """A 3 x BF16 plan on slice planes (one batched cut of every distinct operand, then one
+    grouped level launch over every item's tiles) gives each item the bits of the
+    single-product slice-plane path (phpc_sgemm_device with BF16X3, which takes the planes
+    at m n >= 2048^2 and k >= 256): ragged shapes, k not a multiple of 64, one A shared by
+    two items, C not zero."""
+    import torch
+    A0 = torch.from_numpy(O.uniform((2100, 1000), 31).astype(np.float32)).cuda()
+    ops = [(A0, 1000, 2300), (A0, 1000, 2050)]
+    for i, (m, k, n) in enumerate([(2048, 300, 2048), (2500, 257, 1700)]):
+        ops.append((torch.from_numpy(O.uniform((m, k), 40 + i).astype(np.float32)).cuda(), k, n))
+    items, want = [], []
+    for i, (A, k, n) in enumerate(ops):
+        B = torch.from_numpy(O.uniform((k, n), 50 + i).astype(np.float32)).cuda()
+        C0 = torch.from_numpy(O.uniform((A.shape[0], n), 60 + i).astype(np.float32)).cuda()
+        items.append((A, B, C0.clone()))
+        Cs = C0.clone()
+        phpc.sgemm_device(A, B, Cs, method=phpc.SGEMM_BF16X3)
+        want.append(Cs)
+    plan = _x3_plan(items, cfg)
+    plan.launch()
+    torch.cuda.synchronize()
+    for i, ((_, _, C), w) in enumerate(zip(items, want)):
+        assert torch.equal(C.view(torch.int32), w.view(torch.int32)), i
+    plan.close()
+
+
+SMALL_ITEMS = [(1, 1, 1), (64, 64, 64), (300, 200, 150), (1000, 17, 999), (7, 513, 65), (129, 130, 4100),
+               (256, 256, 64), (33, 1, 700)]
+
+
+@pytest.mark.parametrize("cfg", [4, 6])
+def test_bf16x3_plan_slice_planes_small_items(cfg):
+    """Small and ragged items (1 x 1 x 1 up to k = 4100) on slice planes: exact on
+    small-integer inputs (every product and partial sum an integer below 2^24), within the
+    FP32 bound on uniform ones, and hipGraph replays (eager work between them) equal to the
+    eager launch bit for bit."""
+    import torch
+    rng = np.random.default_rng(8)
+    host, items = [], []
+    for i, (m, n, k) in enumerate(SMALL_ITEMS):
+        A = rng.integers(-8, 9, (m, k)).astype(np.float32)
+        B = rng.integers(-8, 9, (k, n)).astype(np.float32)
+        items.append((torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(),
+                      torch.zeros((m, n), dtype=torch.float32, device="cuda")))
+        host.append((A, B))
+    plan = _x3_plan(items, cfg)
+    plan.launch()
+    torch.cuda.synchronize()
+    for (A, B), (_, _, C) in zip(host, items):
+        want = (A.astype(np.int64) @ B.astype(np.int64)).astype(np.float32)
+        assert np.array_equal(C.cpu().numpy(), want), A.shape
+    plan.close()
+
+    host, items = [], []
+    for i, (m, n, k) in enumerate(SMALL_ITEMS):
+        A = O.uniform((m, k), 70 + i).astype(np.float32)
+        B = O.uniform((k, n), 80 + i).astype(np.float32)
+        C0 = O.uniform((m, n), 90 + i).astype(np.float32)
+        items.append((torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(), torch.from_numpy(C0.copy()).cuda()))
+        host.append((A, B, C0))
+    s = torch.cuda.Stream()
+    plan = _x3_plan(items, cfg, stream=s)
+    eager = None
+    for rep in range(4):
+        for (_, _, C0), (_, _, C) in zip(host, items):
+            C.copy_(torch.from_numpy(C0))
+        torch.cuda.synchronize()
+        with torch.cuda.stream(s):
+            if rep == 0:
+                plan.launch(s)
+            else:
+                plan.graph_launch(s)
+        s.synchronize()
+        _ = torch.ones(1024, device="cuda").sum().item()  # eager work between replays
+        got = [it[2].clone() for it in items]
+        if eager is None:
+            eager = got
+            for (A, B, C0), C in zip(host, got):
+                ref = C0.astype(np.float64) + A.astype(np.float64) @ B.astype(np.float64)
+                err = np.abs(C.cpu().numpy().astype(np.float64) - ref)
+                assert np.all(err <= bound(A, B, C0)), A.shape
+        else:
+            for i, (g, e) in enumerate(zip(got, eager)):
+                assert torch.equal(g.view(torch.int32), e.view(torch.int32)), (rep, i)
+    plan.close()
+
