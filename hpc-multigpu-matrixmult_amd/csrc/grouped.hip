@@ -310,6 +310,12 @@ extern "C" int phpc_plan_create(int dtype, const phpc_gemm_item *items, int coun
     }
   }
   int rc = plan_upload(p, cfg, s);
+  if (rc && dtype == PHPC_F32X3 && cfg >= kF32X3PlanesConfig) {
+    // the slice planes (6 bytes per element of every operand) did not fit: the in-register
+    // kernel needs no workspace
+    cfg = 2;
+    rc = plan_upload(p, cfg, s);
+  }
   if (rc) {
     phpc_plan_destroy(p);
     return rc;

@@ -308,3 +308,25 @@ def test_bf16x3_plan_slice_planes_small_items(cfg):
                 assert torch.equal(g.view(torch.int32), e.view(torch.int32)), (rep, i)
     plan.close()
 
+
+
+def test_bf16x3_plan_falls_back_when_planes_do_not_fit():
+    """A k whose slice-plane rows would pass the level kernel's 32-bit offsets (256 rows x
+    3 planes x 2 bytes x kp >= 2^31) makes a plan pinned to slice planes fall back to the
+    in-register 3 x BF16 kernel (configuration 2), with a result within the bound."""
+    import torch
+    k = 1400000  # kp = k; 256 x 6 x kp > 2^31
+    A = O.uniform((1, k), 5).astype(np.float32)
+    B = O.uniform((k, 3), 6).astype(np.float32)
+    C = torch.zeros((1, 3), dtype=torch.float32, device="cuda")
+    phpc.set_tuning(8, 6)
+    try:
+        plan = phpc.Plan(phpc.F32X3, [(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(), C)])
+    finally:
+        phpc.set_tuning(8, -1)
+    assert plan.info()["config"] == 2
+    plan.launch()
+    torch.cuda.synchronize()
+    ref = A.astype(np.float64) @ B.astype(np.float64)
+    assert np.all(np.abs(C.cpu().numpy() - ref) <= bound(A, B, np.zeros_like(ref)))
+    plan.close()
