@@ -1123,6 +1123,9 @@ struct CrtConsts {
   int qp[kMaxMods][5];  // packed symmetric q[s][t]
   float rmf[kMaxMods];  // 1 / m, rounded to FP32
   uint32_t c8[kMaxMods][3];  // byte i of word w: 2^(8 (4w + i)) mod m, for the 9 bytes of |x| < 2^72
+  // |x| <= 2^46: the six bytes of x's 48-bit two's complement as c8's first six, then
+  // byte 2 of word 1: (-2^48) mod m, the weight of the sign
+  uint32_t c8n[kMaxMods][2];
 };
 static_assert(sizeof(CrtConsts) <= 2560, "CRT constants travel as a kernel argument");
 
@@ -1184,6 +1187,40 @@ __device__ __forceinline__ void residues_of(double x, const CrtConsts &cc, int (
   }
 }
 
+// The same residues for |x| <= 2^46 (p <= kNarrowBits), from x's 48-bit two's complement
+// u = x mod 2^48 and its sign bit s: x = u - s 2^48, so x = sum_i byte_i(u) 2^(8 i) +
+// s (-2^48) (mod m), two int8 dot products with s riding in word 1's byte 2 (sum < 2^19, the
+// same rint argument), and no sign fix-up. u's low byte is x's residue mod 256. hi =
+// floor(x 2^-24) and lo = x - hi 2^24 are exact (|hi| <= 2^22, 0 <= lo < 2^24); bits 8..24
+// of hi are u's bytes 4-5 and, in bit 16, the sign.
+constexpr int kNarrowBits = 46;
+template <int L>
+__device__ __forceinline__ void residues_narrow(double x, const CrtConsts &cc, int (&r)[L]) {
+  const double hd = floor(x * 0x1p-24);
+  const int hi = (int)hd;
+  const uint32_t lo = (uint32_t)(x - hd * 0x1p24);
+  const uint32_t w0 = lo | ((uint32_t)hi << 24);
+  const uint32_t w1 = __builtin_amdgcn_ubfe((uint32_t)hi, 8, 17);
+  r[0] = (int)(int8_t)(w0 & 0xffu);
+#pragma unroll
+  for (int t = 1; t < L; ++t) {
+    uint32_t sum = __builtin_amdgcn_udot4(w0, cc.c8n[t][0], 0u, false);
+    sum = __builtin_amdgcn_udot4(w1, cc.c8n[t][1], sum, false);
+    const float sf = (float)sum;
+    const float q = rintf(sf * cc.rmf[t]);
+    r[t] = (int)fmaf(-q, (float)cc.m[t], sf);
+  }
+}
+
+// residues of x for the first L moduli, by the narrow form where the caller's p allows it
+template <int L, bool NARROW>
+__device__ __forceinline__ void residues_any(double x, const CrtConsts &cc, int (&r)[L]) {
+  if constexpr (NARROW)
+    residues_narrow<L>(x, cc, r);
+  else
+    residues_of<L>(x, cc, r);
+}
+
 __device__ __forceinline__ uint32_t pack4(int a, int b, int c, int d) {
   return (uint32_t)(a & 0xff) | ((uint32_t)(b & 0xff) << 8) | ((uint32_t)(c & 0xff) << 16) | ((uint32_t)(d & 0xff) << 24);
 }
@@ -1194,7 +1231,7 @@ static_assert(128 % kResColsTK == 0 && kResColsTK == 4 * 32 && 256 % kResColsTJ 
 
 // A8 row i: residue plane t at bytes [t*kp, t*kp + k), zero to kp. Each thread handles
 // four consecutive k and stores one 32-bit word per plane.
-template <int L>
+template <int L, bool NARROW>
 __global__ __launch_bounds__(256) void residue_rows_kernel(const double *__restrict__ A, long lda, int k, int p,
                                                            long kp, const int *__restrict__ eA, CrtConsts cc,
                                                            int8_t *__restrict__ A8, long ld8) {
@@ -1208,7 +1245,7 @@ __global__ __launch_bounds__(256) void residue_rows_kernel(const double *__restr
     for (int q = 0; q < 4; ++q) {
       const long l = l0 + q;
       const double x = l < k ? rint(ldexp(row[l], sh)) : 0.0;
-      residues_of<L>(x, cc, r[q]);
+      residues_any<L, NARROW>(x, cc, r[q]);
     }
 #pragma unroll
     for (int t = 0; t < L; ++t)
@@ -1220,7 +1257,7 @@ __global__ __launch_bounds__(256) void residue_rows_kernel(const double *__restr
 // tiles pass through LDS (17 KiB: registers, not LDS, bound the occupancy); each thread
 // then takes four consecutive k of two columns, so 32 lanes write one column's 128 bytes
 // of a plane.
-template <int L>
+template <int L, bool NARROW>
 __global__ __launch_bounds__(256) void residue_cols_kernel(const double *__restrict__ B, long ldb, int k, int n, int p,
                                                            long kp, const int *__restrict__ eB, CrtConsts cc,
                                                            int8_t *__restrict__ B8, long ld8) {
@@ -1241,7 +1278,7 @@ __global__ __launch_bounds__(256) void residue_cols_kernel(const double *__restr
     const int sh = p - eB[jj];
     int r[4][L];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) residues_of<L>(rint(ldexp(tile[kq + q][c], sh)), cc, r[q]);
+    for (int q = 0; q < 4; ++q) residues_any<L, NARROW>(rint(ldexp(tile[kq + q][c], sh)), cc, r[q]);
     int8_t *out = B8 + (long)jj * ld8 + k0 + kq;
 #pragma unroll
     for (int t = 0; t < L; ++t)
@@ -1362,9 +1399,13 @@ struct CrtLaunch {
                  long kp, const int *eA, const int *eB, const CrtConsts &cc, int8_t *A8, int8_t *B8, long ld8,
                  int stage, const int8_t *R8, double *dC, long ldc) {
     if (stage == 0) {
-      hipLaunchKernelGGL(residue_rows_kernel<L>, dim3(m), dim3(256), 0, s, dA, lda, k, p, kp, eA, cc, A8, ld8);
+      // |A'|, |B'| <= 2^p: the narrow residue form takes p <= kNarrowBits
+      const bool narrow = p <= kNarrowBits;
+      auto rows_fn = narrow ? residue_rows_kernel<L, true> : residue_rows_kernel<L, false>;
+      auto cols_fn = narrow ? residue_cols_kernel<L, true> : residue_cols_kernel<L, false>;
+      hipLaunchKernelGGL(rows_fn, dim3(m), dim3(256), 0, s, dA, lda, k, p, kp, eA, cc, A8, ld8);
       PHPC_HIP(hipGetLastError());
-      hipLaunchKernelGGL(residue_cols_kernel<L>,
+      hipLaunchKernelGGL(cols_fn,
                          dim3((unsigned)((kp + kResColsTK - 1) / kResColsTK),
                               (unsigned)((n + kResColsTJ - 1) / kResColsTJ)),
                          dim3(256), 0, s, dB, ldb, k, n, p, kp, eB, cc, B8, ld8);
@@ -1436,6 +1477,8 @@ CrtConsts make_crt(int L) {
     long w = 1;  // 2^(8 i) mod m
     for (int i = 0; i < 9; ++i) {
       c.c8[t][i / 4] |= (uint32_t)w << (8 * (i % 4));
+      if (i < 6) c.c8n[t][i / 4] |= (uint32_t)w << (8 * (i % 4));
+      if (i == 6) c.c8n[t][1] |= (uint32_t)((m - w) % m) << 16;  // w = 2^48 mod m
       w = w * 256 % m;
     }
   }
