@@ -3,7 +3,8 @@
 
 - residues: for every odd modulus and every byte-dot-product sum the kernels can form
   (0 .. 9 * 255^2), q = rint(float32(sum) * float32(1/m)) is the exact nearest quotient,
-  so sum - q*m is the symmetric residue;
+  so sum - q*m is the symmetric residue; the narrow form (|x| <= 2^46, the 48-bit two's
+  complement and its sign byte) gives the same residues;
 - Garner digits: with the coefficients stored in the symmetric (int8) range, the digit sums
   stay below 2^18, the same rint reduction is exact there, and the mixed-radix digits
   rebuild every integer in the symmetric range of the moduli product; the plain-FP64
@@ -47,6 +48,36 @@ def test_byte_weights_give_the_residue_of_the_integer():
             b = [(x >> (8 * i)) & 0xFF for i in range(9)]
             s = sum(bi * wi for bi, wi in zip(b, w))
             assert s <= 9 * 255 * 255
+            assert sym(s, m) == sym(x, m)
+
+
+def test_narrow_two_complement_form_gives_the_residue():
+    """|x| <= 2^46: x = u - s 2^48 with u its 48-bit two's complement and s its sign, so the
+    six bytes of u weighted by 2^(8 i) mod m, plus s weighted by (-2^48) mod m, sum to x mod
+    m; the sum stays below 2^19 (inside the rint argument above), and u's low byte as int8
+    is x's residue mod 256. hi = floor(x 2^-24), lo = x - hi 2^24 are exact in FP64, and
+    bits 8..24 of hi (32-bit two's complement) are u's bytes 4-5 and, in bit 16, s."""
+    rng = np.random.default_rng(4)
+    xs = [0, 1, -1, 2 ** 46, -2 ** 46, 2 ** 46 - 1, 1 - 2 ** 46, 2 ** 24, -2 ** 24, 2 ** 24 - 1, 1 - 2 ** 24, 128, -128,
+          -129, 255, -256, 2 ** 32, -2 ** 32, 2 ** 40 + 7, -2 ** 40 - 7]
+    xs += [int(v) for v in rng.integers(-2 ** 46, 2 ** 46, 3000, endpoint=True)]
+    xs += [int(v) for v in rng.integers(-2 ** 20, 2 ** 20, 1000)]
+    for x in xs:
+        hd = math.floor(float(x) * 2.0 ** -24)
+        lo = float(x) - hd * 2.0 ** 24
+        assert -2 ** 22 <= hd <= 2 ** 22 and 0 <= lo < 2 ** 24 and hd * 2 ** 24 + lo == x
+        hi32 = int(hd) & 0xFFFFFFFF
+        w0 = (int(lo) | (hi32 << 24)) & 0xFFFFFFFF
+        w1 = (hi32 >> 8) & 0x1FFFF
+        u, sgn = x % 2 ** 48, int(x < 0)
+        assert w0 == u & 0xFFFFFFFF and w1 == (u >> 32) | (sgn << 16)
+        low = w0 & 0xFF
+        assert (low - 256 if low > 127 else low) == sym(x, 256)
+        for m in MODULI[1:]:
+            c = [pow(2, 8 * i, m) for i in range(6)]
+            s = sum(((w0 >> (8 * i)) & 0xFF) * c[i] for i in range(4))
+            s += (w1 & 0xFF) * c[4] + ((w1 >> 8) & 0xFF) * c[5] + ((w1 >> 16) & 0xFF) * ((-2 ** 48) % m)
+            assert s < 2 ** 19
             assert sym(s, m) == sym(x, m)
 
 
