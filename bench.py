@@ -328,10 +328,11 @@ def split_sweep(phpc, sizes=(4096, 8192, 16384), seed=31):
     return out
 
 
-def fp32_accurate_bench(phpc, sizes=(4096, 8192, 16384), seed=41):
+def fp32_accurate_bench(phpc, sizes=(4096, 8192, 16384), seed=41, clock=True):
     """FP32 inputs, uniform[-1,1): the FP32 matrix-core GEMM and the FP32-accurate 3 x BF16
     GEMM (six BF16 slice products, FP32 accumulation). Error against float64 dot products
-    of the same FP32 inputs on sampled entries, normalised by |A||B|."""
+    of the same FP32 inputs on sampled entries, normalised by |A||B|. At the largest N, the
+    graphics clock each path holds (clock_during), unless clock is False."""
     import numpy as np
     import torch
 
@@ -367,6 +368,8 @@ def fp32_accurate_bench(phpc, sizes=(4096, 8192, 16384), seed=41):
             row[tag] = {"tflops": round(tf, 3), "ms": round(ms, 3), "err_cw_max_sampled": err,
                         "mfma_util": round(tf * (6 if method == phpc.SGEMM_BF16X3 else 1) /
                                            (2500.0 if method == phpc.SGEMM_BF16X3 else 157.3), 4)}
+            if clock and N == max(sizes):
+                row[tag]["clock_mhz"] = clock_during(lambda: phpc.sgemm_device(A, B, C, method=method))
         row["bf16x3_vs_fp32"] = round(row["bf16x3"]["tflops"] / row["fp32_mfma"]["tflops"], 3)
         out.append(row)
         del A, B, C
@@ -585,7 +588,7 @@ def main():
             res = {"method": args.split_method, "count": info.slices, "gemms": info.products,
                    "int_bits": info.int_bits, "calls": 1 + args.steps, "wall_s": round(time.perf_counter() - t0, 3)}
         elif args.only == "fp32acc":
-            res = fp32_accurate_bench(phpc)
+            res = fp32_accurate_bench(phpc, clock=not args.no_clock)
         elif args.only == "grouped":
             res = grouped_bench(phpc)
         elif args.only == "sweep":
@@ -723,7 +726,7 @@ def main():
         detail["grouped"] = grouped_bench(phpc)
     if not args.no_sweep:
         detail["split_sweep"] = split_sweep(phpc)
-        detail["fp32_accurate"] = fp32_accurate_bench(phpc)
+        detail["fp32_accurate"] = fp32_accurate_bench(phpc, clock=not args.no_clock)
     if not args.no_serve:
         detail["serving"] = serving_bench(phpc, args.serve_requests)
     # the full detail on a line of its own, before the result line
@@ -839,7 +842,10 @@ def main():
     if "fp32_accurate" in detail:
         out["fp32_accurate"] = {str(r["N"]): {"fp32_mfma_tflops": r["fp32_mfma"]["tflops"],
                                               "bf16x3_tflops": r["bf16x3"]["tflops"],
-                                              "bf16x3_err_cw": r["bf16x3"]["err_cw_max_sampled"]}
+                                              "bf16x3_err_cw": r["bf16x3"]["err_cw_max_sampled"],
+                                              **({"clock_mhz": [(r[t]["clock_mhz"] or {}).get("mhz_median")
+                                                                for t in ("fp32_mfma", "bf16x3")]}
+                                                 if "clock_mhz" in r["bf16x3"] else {})}
                                 for r in detail["fp32_accurate"]}
     if "split_sweep" in detail:
         # error against split count at each N (CRT moduli / slices): err_cw and TFLOP/s
