@@ -784,7 +784,7 @@ def main():
         "split_moduli": {"uniform": uni["moduli"], "wide_exponent": wide["moduli"]},
         "split_roofline": {
             "bound": "mfma",
-            "kernel": "i8_level_256x256_w16_k128 (one int8 residue GEMM per modulus)",
+            "kernel": "i8_level_256x256_w16_k128_mi16 (one int8 residue GEMM per modulus, 16x16x64 MFMAs)",
             "achieved": uni["int8_kernel_tops"],
             "peak": 5000.0,
             "unit": "TOP/s",

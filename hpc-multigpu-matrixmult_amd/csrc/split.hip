@@ -33,6 +33,7 @@ namespace {
 typedef int i4v __attribute__((ext_vector_type(4)));
 typedef int i16v __attribute__((ext_vector_type(16)));
 typedef float f16v __attribute__((ext_vector_type(16)));
+typedef float f4v __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 typedef int i8v __attribute__((ext_vector_type(8)));
 typedef __amdgpu_buffer_rsrc_t rsrc_t;
@@ -428,10 +429,17 @@ __device__ __forceinline__ LevelOps level_ops(const LevelArgs &args) {
 }
 
 // One BM x BN output tile at (row0, col0) of the product ops describes; args carries the
-// int8 and FP8 epilogues' fields (the tile walk is the caller's).
-template <int BM, int BN, int WM, int WN, int BK_, int STAGES, int SD, bool BF, bool F8>
+// int8 and FP8 epilogues' fields (the tile walk is the caller's). MI: the MFMA block,
+// 32 x 32 (32 bytes of k per int8 product, 16 per BF16 one) or 16 x 16 (v_mfma_i32_16x16x64_i8
+// and v_mfma_f32_16x16x32_bf16: 64 bytes of k); a lane's 16-byte operand is row l % MI,
+// chunk (64 / MI) kk + l / MI of the k step. Not for FP8 at 16.
+template <int BM, int BN, int WM, int WN, int BK_, int STAGES, int SD, bool BF, bool F8, int MI = 32>
 __device__ __forceinline__ void level_tile(const LevelArgs &args, const LevelOps &ops, int row0, int col0) {
   using K = I8Cfg<BM, BN, WM, WN, BK_, STAGES>;
+  static_assert(MI == 32 || (MI == 16 && !F8), "MFMA block");
+  constexpr int TM = K::WTM / MI, TN = K::WTN / MI;
+  constexpr int KSTEPS = MI == 32 ? K::KSTEPS : K::BK / 64;
+  static_assert(TM >= 1 && TN >= 1 && KSTEPS >= 1, "wave tile");
   extern __shared__ __attribute__((aligned(1024))) char smem[];
   const int tid = threadIdx.x;
   const int wave = uni(tid >> 6);
@@ -451,12 +459,13 @@ __device__ __forceinline__ void level_tile(const LevelArgs &args, const LevelOps
   // 32, and the swizzle of a multiple of 32 is 0, so the slot depends on l and kk alone:
   // fragment i of A sits at a_row + i*32*BK + cofs[kk] (B likewise), the constant part
   // folding into the LDS instruction's offset.
-  const int sw = i8_swz<K::CPR, SD>(lane & 31);
-  const int a_row = (wm * K::WTM + (lane & 31)) * K::BK;
-  const int b_row = K::A_BYTES + (wn * K::WTN + (lane & 31)) * K::BK;
-  int cofs[K::KSTEPS];
+  // (MI = 16: rows l % 16 plus multiples of 16, whose swizzle is 0 too)
+  const int sw = i8_swz<K::CPR, SD>(lane & (MI - 1));
+  const int a_row = (wm * K::WTM + (lane & (MI - 1))) * K::BK;
+  const int b_row = K::A_BYTES + (wn * K::WTN + (lane & (MI - 1))) * K::BK;
+  int cofs[KSTEPS];
 #pragma unroll
-  for (int kk = 0; kk < K::KSTEPS; ++kk) cofs[kk] = ((2 * kk + (lane >> 5)) ^ sw) << 4;
+  for (int kk = 0; kk < KSTEPS; ++kk) cofs[kk] = (((64 / MI) * kk + lane / MI) ^ sw) << 4;
 
   {
     const int m_rem = m - row0, n_rem = n - col0;
@@ -516,12 +525,14 @@ __device__ __forceinline__ void level_tile(const LevelArgs &args, const LevelOps
       }
     };
 
-    using acc_t = typename std::conditional<BF || F8, f16v, i16v>::type;
-    acc_t acc[K::TM][K::TN];
+    using acc32_t = typename std::conditional<BF || F8, f16v, i16v>::type;
+    using acc16_t = typename std::conditional<BF, f4v, i4v>::type;
+    using acc_t = typename std::conditional<MI == 32, acc32_t, acc16_t>::type;
+    acc_t acc[TM][TN];
 #pragma unroll
-    for (int i = 0; i < K::TM; ++i)
+    for (int i = 0; i < TM; ++i)
 #pragma unroll
-      for (int j = 0; j < K::TN; ++j) acc[i][j] = acc_t{};
+      for (int j = 0; j < TN; ++j) acc[i][j] = acc_t{};
 
     slab_barrier();
 #pragma unroll
@@ -569,40 +580,50 @@ __device__ __forceinline__ void level_tile(const LevelArgs &args, const LevelOps
         }
       } else {
 #pragma unroll
-        for (int kk = 0; kk < K::KSTEPS; ++kk) {
-          i4v a[K::TM], b[K::TN];
+        for (int kk = 0; kk < KSTEPS; ++kk) {
+          i4v a[TM], b[TN];
 #pragma unroll
-          for (int i = 0; i < K::TM; ++i)
-            a[i] = *reinterpret_cast<const i4v *>(stage + a_row + cofs[kk] + i * 32 * K::BK);
+          for (int i = 0; i < TM; ++i)
+            a[i] = *reinterpret_cast<const i4v *>(stage + a_row + cofs[kk] + i * MI * K::BK);
 #pragma unroll
-          for (int j = 0; j < K::TN; ++j)
-            b[j] = *reinterpret_cast<const i4v *>(stage + b_row + cofs[kk] + j * 32 * K::BK);
+          for (int j = 0; j < TN; ++j)
+            b[j] = *reinterpret_cast<const i4v *>(stage + b_row + cofs[kk] + j * MI * K::BK);
 #pragma unroll
-          for (int i = 0; i < K::TM; ++i)
+          for (int i = 0; i < TM; ++i)
 #pragma unroll
-            for (int j = 0; j < K::TN; ++j)
-              if constexpr (BF)
+            for (int j = 0; j < TN; ++j)
+              if constexpr (BF && MI == 32)
                 acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8v, a[i]),
                                                                    __builtin_bit_cast(bf8v, b[j]), acc[i][j], 0, 0, 0);
-              else
+              else if constexpr (BF)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8v, a[i]),
+                                                                   __builtin_bit_cast(bf8v, b[j]), acc[i][j], 0, 0, 0);
+              else if constexpr (MI == 32)
                 acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[i], b[j], acc[i][j], 0, 0, 0);
+              else
+                acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], b[j], acc[i][j], 0, 0, 0);
         }
       }
     }
 
-    // epilogue: 32x32 block, lane l / reg r -> row 8(r/4) + 4(l/32) + r%4, column l%32
+    // epilogue: 32x32 block, lane l / reg r -> row 8(r/4) + 4(l/32) + r%4, column l%32;
+    // 16x16 block: row 4(l/16) + r, column l%16
+    constexpr int NR = MI == 32 ? 16 : 4;
+    auto erow = [&](int i, int r) {
+      return MI == 32 ? i * 32 + 8 * (r / 4) + 4 * (lane >> 5) + (r % 4) : i * 16 + 4 * (lane >> 4) + r;
+    };
     if constexpr (BF) {
       float __attribute__((address_space(1))) *Cg = (float __attribute__((address_space(1))) *)ops.C32;
       const long ldc32 = ops.ldc32;
 #pragma unroll
-      for (int i = 0; i < K::TM; ++i)
+      for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int j = 0; j < K::TN; ++j) {
-          const int col = col0 + wn * K::WTN + j * 32 + (lane & 31);
+        for (int j = 0; j < TN; ++j) {
+          const int col = col0 + wn * K::WTN + j * MI + (lane & (MI - 1));
           if (col >= n) continue;
 #pragma unroll
-          for (int r = 0; r < 16; ++r) {
-            const int row = row0 + wm * K::WTM + i * 32 + 8 * (r / 4) + 4 * (lane >> 5) + (r % 4);
+          for (int r = 0; r < NR; ++r) {
+            const int row = row0 + wm * K::WTM + erow(i, r);
             if (row >= m) continue;
             auto *pc = Cg + (long)row * ldc32 + col;
             *pc = *pc + acc[i][j][r];
@@ -617,14 +638,14 @@ __device__ __forceinline__ void level_tile(const LevelArgs &args, const LevelOps
       int8_t *resg = uni(args.res);
       const long ldres = uni(args.ldres);
 #pragma unroll
-      for (int i = 0; i < K::TM; ++i)
+      for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int j = 0; j < K::TN; ++j) {
-          const int col = col0 + wn * K::WTN + j * 32 + (lane & 31);
+        for (int j = 0; j < TN; ++j) {
+          const int col = col0 + wn * K::WTN + j * MI + (lane & (MI - 1));
           if (col >= n) continue;
 #pragma unroll
-          for (int r = 0; r < 16; ++r) {
-            const int row = row0 + wm * K::WTM + i * 32 + 8 * (r / 4) + 4 * (lane >> 5) + (r % 4);
+          for (int r = 0; r < NR; ++r) {
+            const int row = row0 + wm * K::WTM + erow(i, r);
             if (row >= m) continue;
             int8_t *pr = resg + (long)row * ldres + col;
             double v = (double)acc[i][j][r];
@@ -640,15 +661,15 @@ __device__ __forceinline__ void level_tile(const LevelArgs &args, const LevelOps
     gdouble *Cg = (gdouble *)uni(args.C);
     const long ldacc = uni(args.ldacc), ldc = uni(args.ldc);
 #pragma unroll
-    for (int i = 0; i < K::TM; ++i)
+    for (int i = 0; i < TM; ++i)
 #pragma unroll
-      for (int j = 0; j < K::TN; ++j) {
-        const int col = col0 + wn * K::WTN + j * 32 + (lane & 31);
+      for (int j = 0; j < TN; ++j) {
+        const int col = col0 + wn * K::WTN + j * MI + (lane & (MI - 1));
         if (col >= n) continue;
         const int ej = last ? args.eB[col] : 0;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const int row = row0 + wm * K::WTM + i * 32 + 8 * (r / 4) + 4 * (lane >> 5) + (r % 4);
+        for (int r = 0; r < NR; ++r) {
+          const int row = row0 + wm * K::WTM + erow(i, r);
           if (row >= m) continue;
           double v = ldexp((double)acc[i][j][r], sc);
           gdouble *pa = accg + (long)row * ldacc + col;
@@ -664,7 +685,8 @@ __device__ __forceinline__ void level_tile(const LevelArgs &args, const LevelOps
   }
 }
 
-template <int BM, int BN, int WM, int WN, int BK_, int STAGES, int SD = 16, bool BF = false, bool F8 = false>
+template <int BM, int BN, int WM, int WN, int BK_, int STAGES, int SD = 16, bool BF = false, bool F8 = false,
+          int MI = 32>
 __device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
   const LevelOps ops = level_ops<BK_, BF>(args);
   const int m = ops.m, n = ops.n;
@@ -685,7 +707,7 @@ __device__ __forceinline__ void i8_level_body(const LevelArgs &args) {
     const int in_grp = t - grp * gm * tiles_n;
     const int row0 = uni((first_m + in_grp % gsz) * BM);
     const int col0 = uni((in_grp / gsz) * BN);
-    level_tile<BM, BN, WM, WN, BK_, STAGES, SD, BF, F8>(args, ops, row0, col0);
+    level_tile<BM, BN, WM, WN, BK_, STAGES, SD, BF, F8, MI>(args, ops, row0, col0);
   }
 }
 
@@ -719,6 +741,16 @@ __global__ __launch_bounds__(1024) void i8_level_256x256_w16_k64_s5(LevelArgs a)
 __global__ __launch_bounds__(1024) void i8_level_256x128_w16_k128_s3(LevelArgs a) {
   i8_level_body<256, 128, 8, 2, 128, 3>(a);
 }
+// the default tile on 16 x 16 MFMA blocks (v_mfma_i32_16x16x64_i8)
+__global__ __launch_bounds__(1024) void i8_level_256x256_w16_k128_mi16(LevelArgs a) {
+  i8_level_body<256, 256, 4, 4, 128, 2, 16, false, false, 16>(a);
+}
+__global__ __launch_bounds__(512) void i8_level_256x256_w8_k128_mi16(LevelArgs a) {
+  i8_level_body<256, 256, 2, 4, 128, 2, 16, false, false, 16>(a);
+}
+__global__ __launch_bounds__(256) void i8_level_128x128_w4_mi16(LevelArgs a) {
+  i8_level_body<128, 128, 2, 2, 64, 4, 16, false, false, 16>(a);
+}
 // FP8-E4M3 digit slices on the same pipeline (FP64-accurate, scheme I with 4-bit digits)
 __global__ __launch_bounds__(1024) void f8_level_256x256_w16_k128(LevelArgs a) {
   i8_level_body<256, 256, 4, 4, 128, 2, 16, false, true>(a);
@@ -726,15 +758,18 @@ __global__ __launch_bounds__(1024) void f8_level_256x256_w16_k128(LevelArgs a) {
 __global__ __launch_bounds__(256) void f8_level_128x128_w4(LevelArgs a) {
   i8_level_body<128, 128, 2, 2, 64, 4, 16, false, true>(a);
 }
-// BF16 slices on the same pipeline (FP32-accurate 3 x BF16 GEMM)
+// BF16 slices on the same pipeline (FP32-accurate 3 x BF16 GEMM). Every BF16 level kernel,
+// single-product and grouped, uses the same MFMA block, so that they sum each output in the
+// same order.
+constexpr int kBfMI = 16;
 __global__ __launch_bounds__(512) void bf16_level_256x256_w8_k128(LevelArgs a) {
-  i8_level_body<256, 256, 2, 4, 128, 2, 16, true>(a);
+  i8_level_body<256, 256, 2, 4, 128, 2, 16, true, false, kBfMI>(a);
 }
 __global__ __launch_bounds__(1024) void bf16_level_256x256_w16_k128(LevelArgs a) {
-  i8_level_body<256, 256, 4, 4, 128, 2, 16, true>(a);
+  i8_level_body<256, 256, 4, 4, 128, 2, 16, true, false, kBfMI>(a);
 }
 __global__ __launch_bounds__(256) void bf16_level_128x128_w4(LevelArgs a) {
-  i8_level_body<128, 128, 2, 2, 64, 4, 16, true>(a);
+  i8_level_body<128, 128, 2, 2, 64, 4, 16, true, false, kBfMI>(a);
 }
 
 typedef void (*LevelFn)(LevelArgs);
@@ -754,14 +789,17 @@ const I8Config kI8Configs[] = {
     I8CFG(256, 256, 2, 4, 64, 5, i8_level_256x256_w8_s5),         // 5: 8 waves, 5-stage pipeline
     I8CFG(256, 256, 2, 4, 128, 2, i8_level_256x256_w8_k128_sd8),  // 6: as 2, 32-bank swizzle
     I8CFG(256, 256, 2, 4, 64, 3, i8_level_256x256_w8_sd8),        // 7: as 0, 32-bank swizzle
-    I8CFG(256, 256, 4, 4, 128, 2, i8_level_256x256_w16_k128),     // 8: 16 waves, 64x64 wave tiles (default)
+    I8CFG(256, 256, 4, 4, 128, 2, i8_level_256x256_w16_k128),     // 8: 16 waves, 64x64 wave tiles
     I8CFG(256, 128, 4, 2, 128, 3, i8_level_256x128_w8_k128_s3),   // 9: 8 waves, 3 stages of 128-byte slabs
     I8CFG(256, 256, 4, 4, 64, 4, i8_level_256x256_w16_k64_s4),    // 10: 16 waves, 4 stages of 64-byte slabs
     I8CFG(256, 256, 4, 4, 64, 5, i8_level_256x256_w16_k64_s5),    // 11: 16 waves, 5 stages of 64-byte slabs
     I8CFG(256, 128, 8, 2, 128, 3, i8_level_256x128_w16_k128_s3),  // 12: 16 waves (32x64 wave tiles), 3 stages
+    I8CFG(256, 256, 4, 4, 128, 2, i8_level_256x256_w16_k128_mi16),  // 13: as 8 on 16x16 MFMA blocks (default)
+    I8CFG(256, 256, 2, 4, 128, 2, i8_level_256x256_w8_k128_mi16),   // 14: as 2 on 16x16 MFMA blocks
+    I8CFG(128, 128, 2, 2, 64, 4, i8_level_128x128_w4_mi16),         // 15: as 1 on 16x16 MFMA blocks
 };
 constexpr int kNumI8 = sizeof(kI8Configs) / sizeof(kI8Configs[0]);
-int g_i8_big = 8;  // kI8Configs index for large problems
+int g_i8_big = 13;  // kI8Configs index for large problems
 
 // BF16-slice level kernels (FP32-accurate 3 x BF16 GEMM), kept apart from the int8
 // table so that phpc_split_set_config can only pick int8 kernels
@@ -1047,7 +1085,7 @@ __device__ __forceinline__ void bf16_group_body(const Bf3Group &g) {
     const int in_grp = lt - grp * gm * tiles_n;
     const int row0 = uni((first_m + in_grp % gsz) * BM);
     const int col0 = uni((in_grp / gsz) * BN);
-    level_tile<BM, BN, WM, WN, BK_, STAGES, 16, true, false>(none, o, row0, col0);
+    level_tile<BM, BN, WM, WN, BK_, STAGES, 16, true, false, kBfMI>(none, o, row0, col0);
   }
 }
 __global__ __launch_bounds__(256) void bf16_group_128x128_w4(Bf3Group g) { bf16_group_body<128, 128, 2, 2, 64, 4>(g); }

@@ -399,9 +399,9 @@ def test_int8_level_configs_bit_identical():
             results[cfg] = dC
             cfg += 1
     finally:
-        phpc.lib().phpc_split_set_config(8)
-    assert len(results) >= 13
-    ref = results[8]
+        phpc.lib().phpc_split_set_config(13)  # the default large-problem kernel
+    assert len(results) >= 16
+    ref = results[13]
     for c, dC in results.items():
         assert torch.equal(dC.view(torch.int64), ref.view(torch.int64)), c
     rng = np.random.default_rng(6)
