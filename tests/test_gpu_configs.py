@@ -6,7 +6,8 @@
   over the whole matrix against the register-staged kernel;
 - the grouped N = 10000 3 x 3 plan (rank blocks 3334/3333/3333), sampled bit for bit;
 - the split-precision GEMM at N = 8192 and 16384 on both input sets, sampled against the
-  double-double product;
+  double-double product, and its FP32-accurate 3 x BF16 variant at the same sizes within
+  its bound;
 - 512 FP32 GEMMs with M, N, K log-uniform in [64, 4096] in one plan, every GEMM within
   the FP32 bound, hipGraph replays equal to the eager launch; the same GEMMs as a 3 x BF16
   plan on slice planes within the 3 x BF16 bound;
@@ -192,6 +193,31 @@ def f32_sampled_ok(A, B, C, count, seed, C0=None, extra=0.0):
     err = (C[r, c].double() - ref).abs()
     bound = ((k + 2) * 2.0 ** -24 + extra) * mag
     return bool(torch.all(err <= bound)), float((err / mag.clamp_min(1e-300)).max())
+
+
+@pytest.mark.parametrize("N", [8192, 16384])
+def test_fp32_accurate_bf16x3_config_sizes(N):
+    """The FP32-accurate 3 x BF16 variant of the split-precision workload at its benchmark
+    sizes, FP32 uniform inputs: 512 sampled entries within the 3 x BF16 bound
+    ((k + 2) 2^-24 + 2^-23) (|C0| + |A||B|), C not zero, and a second call on the same
+    inputs gives the same bits."""
+    import torch
+    print(f"bf16x3 N={N}", flush=True)
+    A = torch.empty((N, N), dtype=torch.float32, device="cuda")
+    B = torch.empty_like(A)
+    C0 = torch.empty_like(A)
+    phpc.fill_device(A, phpc.DIST_UNIFORM, 41)
+    phpc.fill_device(B, phpc.DIST_UNIFORM, 42)
+    phpc.fill_device(C0, phpc.DIST_UNIFORM, 43)
+    C = C0.clone()
+    phpc.sgemm_device(A, B, C, method=phpc.SGEMM_BF16X3)
+    C2 = C0.clone()
+    phpc.sgemm_device(A, B, C2, method=phpc.SGEMM_BF16X3)
+    torch.cuda.synchronize()
+    ok, rel = f32_sampled_ok(A, B, C, 512, N, C0=C0, extra=2.0 ** -23)
+    print(f"  worst err_cw {rel:.2e}", flush=True)
+    assert ok
+    assert torch.equal(C.view(torch.int32), C2.view(torch.int32))
 
 
 def test_grouped_fp32_512_gemms_config_law():
