@@ -344,21 +344,23 @@ struct LevelArgs {
   int segb1, segb2;
 };
 
-// x mod m in the symmetric range (m odd: |r| <= (m-1)/2; m = 256: [-128, 127]) for
-// integer-valued doubles |x| < 2^40, with rm = 1/m rounded: the quotient estimate
-// floor(x * rm) is off by at most one (|x/m| * 2^-52 < 2^-19), so one correction step
-// and an exact remainder x - q*m give the residue without a division.
-__device__ __forceinline__ int smod_r(double x, int m, double rm) {
-  const double md = (double)m;
-  const double q = floor(x * rm);
-  double r = x - q * md;  // exact: |q * m| < 2^41
-  if (r < 0.0) r += md;
-  if (r >= md) r -= md;
-  int ri = (int)r;
-  if (ri > (m - 1) / 2) ri -= m;
-  return ri;
+// v mod m in the symmetric range for |v| < 2^31 and a wave-uniform m: m = 256 gives
+// [-128, 127] (v's low byte), an odd m <= 255 gives |r| <= (m-1)/2, in int32 and FP32
+// arithmetic with rmf = 1/m rounded to FP32, on a = |v| (the residue of -a is minus that
+// of a for odd m). First round: float(a) is within 64 of a and the rounded product within
+// about 2 of a/m (< 2^24), so q1 = rint(a * rmf) leaves an exact remainder a - q1 m (a
+// 24-bit unsigned multiply) with |r| < 3m < 2^11. Second round: r is exact in FP32, r * rmf
+// is within 2^-20 of r/m, which for odd m lies at least 1/(2m) from a half-integer, so
+// q2 = rint(r * rmf) is the nearest quotient and r - q2 m the symmetric residue.
+__device__ __forceinline__ int smod_i32(int v, int m, float rmf) {
+  if (m == 256) return (int)(int8_t)(v & 0xff);
+  const uint32_t a = (uint32_t)(v < 0 ? -v : v);
+  const uint32_t q1 = (uint32_t)rintf((float)a * rmf);
+  const int r = (int)(a - __umul24(q1, (uint32_t)m));
+  const int q2 = (int)rintf((float)r * rmf);
+  const int s = r - __mul24(q2, m);
+  return v < 0 ? -s : s;
 }
-__device__ __forceinline__ int smod_d(double x, int m) { return smod_r(x, m, 1.0 / (double)m); }
 
 // LDS layout per stage: A then B, BM (BN) rows of BK k-bytes each, in 16-byte chunks.
 // Chunk c of row r sits at slot c ^ sw(r), sw(r) = (r * CPR / 16) mod CPR (CPR chunks per
@@ -634,7 +636,7 @@ __device__ __forceinline__ void level_tile(const LevelArgs &args, const LevelOps
     const int mod_m = uni(args.mod_m);
     if (mod_m > 0) {
       const int firstr = uni(args.first);
-      const double rmod = 1.0 / (double)mod_m;
+      const float rmf = 1.0f / (float)mod_m;
       int8_t *resg = uni(args.res);
       const long ldres = uni(args.ldres);
 #pragma unroll
@@ -648,9 +650,11 @@ __device__ __forceinline__ void level_tile(const LevelArgs &args, const LevelOps
             const int row = row0 + wm * K::WTM + erow(i, r);
             if (row >= m) continue;
             int8_t *pr = resg + (long)row * ldres + col;
-            double v = (double)acc[i][j][r];
-            if (!firstr) v += (double)*pr;
-            *pr = (int8_t)smod_r(v, mod_m, rmod);
+            // |sum| <= 128^2 x 130944 < 2^31 - 128 (kMaxExactTermsCrt), so adding the
+            // previous piece's residue stays in int32
+            int v = (int)acc[i][j][r];
+            if (!firstr) v += *pr;
+            *pr = (int8_t)smod_i32(v, mod_m, rmf);
           }
         }
       return;

@@ -1,6 +1,7 @@
 """The exact-arithmetic steps of the CRT split GEMM's residue and reconstruction kernels
 (csrc/split.hip), checked on the CPU with IEEE float32 arithmetic as the device does it:
 
+- the residue GEMM's epilogue: int32 sums reduced mod m in two float32 rounds;
 - residues: for every odd modulus and every byte-dot-product sum the kernels can form
   (0 .. 9 * 255^2), q = rint(float32(sum) * float32(1/m)) is the exact nearest quotient,
   so sum - q*m is the symmetric residue; the narrow form (|x| <= 2^46, the 48-bit two's
@@ -79,6 +80,36 @@ def test_narrow_two_complement_form_gives_the_residue():
             s += (w1 & 0xFF) * c[4] + ((w1 >> 8) & 0xFF) * c[5] + ((w1 >> 16) & 0xFF) * ((-2 ** 48) % m)
             assert s < 2 ** 19
             assert sym(s, m) == sym(x, m)
+
+
+def test_epilogue_residue_of_int32_sums():
+    """The residue GEMM's epilogue reduces an int32 sum v (|v| <= 128^2 x 130944 + 128 <
+    2^31) in two float32 rounds on a = |v|: q1 = rint(float32(a) * float32(1/m)) (below
+    2^24, for the kernel's 24-bit multiply), r = a - q1 m (exact), then
+    q2 = rint(float32(r) * float32(1/m)), r - q2 m, with v's sign: the symmetric residue for
+    every odd modulus; m = 256 takes v's low byte."""
+    rng = np.random.default_rng(11)
+    vmax = 128 * 128 * 130944 + 128
+    assert vmax < 2 ** 31
+    v = np.concatenate([rng.integers(-vmax, vmax, 200000, endpoint=True),
+                        np.arange(-3000, 3000), np.array([vmax, -vmax, vmax - 1, 1 - vmax, 2 ** 30, -2 ** 30])])
+    v = v.astype(np.int64)
+    a = np.abs(v)
+    for m in MODULI[1:]:
+        rmf = np.float32(1.0) / np.float32(m)
+        q1 = np.rint((a.astype(np.float32) * rmf).astype(np.float32)).astype(np.int64)
+        assert np.all(q1 < 2 ** 24)  # the kernel's 24-bit unsigned multiply
+        r = a - q1 * m
+        assert np.all(np.abs(r) < 3 * m)
+        q2 = np.rint((r.astype(np.float32) * rmf).astype(np.float32)).astype(np.int64)
+        res = np.where(v < 0, -(r - q2 * m), r - q2 * m)
+        want = v % m
+        want = np.where(want > (m - 1) // 2, want - m, want)
+        assert np.array_equal(res, want), m
+    low = (v & 0xFF).astype(np.int64)
+    low = np.where(low > 127, low - 256, low)
+    want = v % 256
+    assert np.array_equal(low, np.where(want > 127, want - 256, want))
 
 
 def garner_constants(L):
