@@ -231,6 +231,22 @@ def test_crt_matches_restatement_bitwise(m, k, n, L, gen):
     assert np.array_equal(bits(dC.cpu().numpy()), bits(want))
 
 
+@pytest.mark.parametrize("gen", ["u", "w"])
+def test_crt_k_pieces_match_restatement(gen):
+    """k = 140000 is more than one int32 accumulation of residue products holds (130944),
+    so every residue GEMM runs in two k pieces, the second adding the first's residue before
+    it reduces mod m: the result equals the restatement bit for bit."""
+    import torch
+    m, k, n, L = 6, 140000, 5, 12
+    g = O.uniform if gen == "u" else O.wide
+    A, B, C0 = g((m, k), 61), g((k, n), 62), O.uniform((m, n), 63)
+    dC = dev(C0)
+    info = phpc.dgemm_split_device(dev(A), dev(B), dC, slices=L, method=phpc.SPLIT_CRT)
+    torch.cuda.synchronize()
+    want = O.crt_gemm(A, B, C0.copy(), info.int_bits, L)
+    assert np.array_equal(bits(dC.cpu().numpy()), bits(want))
+
+
 MODULI = [256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193, 191, 181, 179, 173]
 
 
