@@ -1345,7 +1345,7 @@ __device__ __forceinline__ double crt_value(const int (&r)[L], const CrtConsts &
     // y / m_t is at least 1 / (2 m_t) from a half-integer, far beyond the FP32 rounding of
     // y * (1 / m_t) (< 2^-12): rint gives the exact nearest quotient and the symmetric
     // residue y - q m_t is exact.
-    int y = r[t] * cc.invs[t];
+    int y = __mul24(r[t], cc.invs[t]);  // |r|, |inv| <= 128: a full-rate 24-bit multiply
 #pragma unroll
     for (int w = 0; w < (t + 3) / 4; ++w) y = __builtin_amdgcn_sdot4((int)vp[w], cc.qp[t][w], y, false);
     const float yf = (float)y;
@@ -1365,7 +1365,7 @@ __device__ __forceinline__ double crt_value(const int (&r)[L], const CrtConsts &
   int t = L - 2;
 #pragma unroll
   for (; t - 1 >= S; t -= 2)
-    hi = fma(hi, (double)(cc.m[t] * cc.m[t - 1]), (double)(v[t] * cc.m[t - 1] + v[t - 1]));
+    hi = fma(hi, (double)(cc.m[t] * cc.m[t - 1]), (double)(__mul24(v[t], cc.m[t - 1]) + v[t - 1]));
   if (t >= S) {
     hi = fma(hi, (double)cc.m[t], (double)v[t]);
     --t;
@@ -1383,7 +1383,7 @@ __device__ __forceinline__ double crt_value(const int (&r)[L], const CrtConsts &
     lo = tl - (hi - sh);
   };
 #pragma unroll
-  for (; t - 1 >= F; t -= 2) dd_step((double)(cc.m[t] * cc.m[t - 1]), (double)(v[t] * cc.m[t - 1] + v[t - 1]));
+  for (; t - 1 >= F; t -= 2) dd_step((double)(cc.m[t] * cc.m[t - 1]), (double)(__mul24(v[t], cc.m[t - 1]) + v[t - 1]));
 #pragma unroll
   for (; t >= 0; --t) dd_step((double)cc.m[t], (double)v[t]);
   return hi + lo;
