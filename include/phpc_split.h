@@ -84,7 +84,9 @@ int phpc_dgemm_split_device_noalloc(const double *dA, long lda, const double *dB
  * largest row scale 2^e_i over the row's mean |a| (ratio_b likewise for columns). */
 double phpc_split_error_estimate(int slices, double ratio_a, double ratio_b, int k);
 
-/* int8 level-GEMM tile configuration for large problems (autotuner knob). */
+/* int8 level-GEMM tile configuration for large problems (autotuner knob; 0 ..
+ * phpc_split_num_configs() - 1, default 13: 256x256 tiles, 16 waves, 16x16x64 MFMAs). Every
+ * configuration gives the same bits. */
 int phpc_split_set_config(int cfg);
 int phpc_split_num_configs(void);
 
