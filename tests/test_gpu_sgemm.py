@@ -330,3 +330,29 @@ def test_bf16x3_plan_falls_back_when_planes_do_not_fit():
     ref = A.astype(np.float64) @ B.astype(np.float64)
     assert np.all(np.abs(C.cpu().numpy() - ref) <= bound(A, B, np.zeros_like(ref)))
     plan.close()
+
+
+def test_bf16x3_plan_slice_planes_strided_operands():
+    """Slice-plane plans on operand views: A and B as column windows of wider matrices
+    (lda > k with rows not 16-byte aligned, so the cut takes its 4-byte path; ldb > n) and
+    C a window too. Each item equals the single-product slice-plane path on the same views
+    bit for bit, and nothing outside the C windows changes."""
+    import torch
+    specs = [(2100, 2300, 1000, 3, 5), (2048, 2049, 300, 1, 7)]
+    items, want, guards = [], [], []
+    for i, (m, n, k, pa, pb) in enumerate(specs):
+        Aw = torch.from_numpy(O.uniform((m, k + pa), 100 + i).astype(np.float32)).cuda()
+        Bw = torch.from_numpy(O.uniform((k, n + pb), 110 + i).astype(np.float32)).cuda()
+        Cw = torch.from_numpy(O.uniform((m, n + 2), 120 + i).astype(np.float32)).cuda()
+        A, B = Aw[:, pa:], Bw[:, :n]
+        Cs = Cw.clone()
+        phpc.sgemm_device(A, B, Cs[:, 1:n + 1], method=phpc.SGEMM_BF16X3)
+        want.append(Cs)
+        items.append((A, B, Cw[:, 1:n + 1]))
+        guards.append(Cw)
+    plan = _x3_plan(items, 6)
+    plan.launch()
+    torch.cuda.synchronize()
+    for i, (Cw, w) in enumerate(zip(guards, want)):
+        assert torch.equal(Cw.view(torch.int32), w.view(torch.int32)), i
+    plan.close()
