@@ -565,14 +565,19 @@ __global__ __launch_bounds__(256) void f32_transpose_kernel(const float *__restr
   __shared__ float tile[T][T + 1];
   const int k0 = blockIdx.x * T, m0 = blockIdx.y * T;
   const int tx = threadIdx.x % T, ty = threadIdx.x / T;
+  constexpr int RS = 256 / T;  // row step
+  float v[T / RS];
 #pragma unroll
-  for (int r = ty; r < T; r += 256 / T) {
-    const int m = m0 + r, k = k0 + tx;
-    tile[r][tx] = (m < M && k < Kd) ? A[(long)m * lda + k] : 0.0f;
+  for (int u = 0; u < T / RS; ++u) {
+    const int m = m0 + ty + u * RS, k = k0 + tx;
+    v[u] = load_or_zero(A, (long)m * lda + k, m < M && k < Kd);
   }
+#pragma unroll
+  for (int u = 0; u < T / RS; ++u) tile[ty + u * RS][tx] = v[u];
   __syncthreads();
 #pragma unroll
-  for (int r = ty; r < T; r += 256 / T) {
+  for (int u = 0; u < T / RS; ++u) {
+    const int r = ty + u * RS;
     const int k = k0 + r, m = m0 + tx;
     if (k < Kd && m < ldat) At[(long)k * ldat + m] = tile[tx][r];
   }
@@ -589,12 +594,14 @@ __global__ __launch_bounds__(256) void f32_stage_kernel(const F32StageCopy *__re
   const int d0 = (lb / c.tiles_x) * 64;  // first destination row (k) of the tile
   const int e0 = (lb % c.tiles_x) * 64;  // first destination column (m)
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  float v[16];
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
-    const int r = ty + 4 * i;
-    const int sr = e0 + r, sc = d0 + tx;
-    tile[r][tx] = (sr < c.rows && sc < c.cols) ? c.src[(long)sr * c.ld + sc] : 0.0f;
+    const int sr = e0 + ty + 4 * i, sc = d0 + tx;
+    v[i] = load_or_zero(c.src, (long)sr * c.ld + sc, sr < c.rows && sc < c.cols);
   }
+#pragma unroll
+  for (int i = 0; i < 16; ++i) tile[ty + 4 * i][tx] = v[i];
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < 16; ++i) {

@@ -899,11 +899,14 @@ __global__ __launch_bounds__(256) void f64_transpose_kernel(const double *__rest
   __shared__ double tile[32][33];
   const int k0 = blockIdx.x * 32, m0 = blockIdx.y * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  double v[4];
 #pragma unroll
-  for (int r = ty; r < 32; r += 8) {
-    const int m = m0 + r, k = k0 + tx;
-    tile[r][tx] = (m < M && k < Kd) ? A[(long)m * lda + k] : 0.0;
+  for (int u = 0; u < 4; ++u) {
+    const int m = m0 + ty + 8 * u, k = k0 + tx;
+    v[u] = load_or_zero(A, (long)m * lda + k, m < M && k < Kd);
   }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) tile[ty + 8 * u][tx] = v[u];
   __syncthreads();
 #pragma unroll
   for (int r = ty; r < 32; r += 8) {

@@ -24,6 +24,16 @@ struct ScopedEvent {
   ScopedEvent(ScopedEvent &&o) noexcept : e(o.e) { o.e = nullptr; }
 };
 
+// p[idx] where ok, else 0, through a load that always happens (from p[0] where !ok; p[0]
+// must be readable). A guarded load compiles to a branch with a wait for the load inside
+// it, so a thread's guarded loads go out one at a time; these all go out before the first
+// wait.
+template <class T>
+__device__ __forceinline__ T load_or_zero(const T *p, long idx, bool ok) {
+  const T v = p[ok ? idx : 0];
+  return ok ? v : T(0);
+}
+
 // ---------------------------------------------------------------------------
 // Error handling: every entry point records its status in a thread-local slot
 // (phpc_last_error) and returns it where the signature allows.

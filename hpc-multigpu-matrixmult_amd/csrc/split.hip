@@ -119,14 +119,25 @@ __global__ __launch_bounds__(256) void row_stats_kernel(const double *__restrict
   const double *row = A + (long)i * lda;
   double mx = 0.0, sm = 0.0;
   int code = 0;
-  for (int l = threadIdx.x; l < k; l += 256) {
-    const double x = row[l];
-    const double v = fabs(x);
-    if (!(v <= 1.79769313486231570815e308)) *bad = 1;  // inf or nan
-    mx = fmax(mx, v);
-    sm += v;
-    code = max(code, low_bit_code(x));
+  bool nonfinite = false;
+  // four loads in flight per thread; the zeros past k change neither the maximum, the
+  // sum (+0.0 added to a sum of magnitudes) nor the code, so the order of the additions,
+  // and the bits, are those of one element at a time
+  for (int l = threadIdx.x; l < k; l += 1024) {
+    double xs[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) xs[q] = load_or_zero(row, l + 256 * q, l + 256 * q < k);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double x = xs[q];
+      const double v = fabs(x);
+      nonfinite |= !(v <= 1.79769313486231570815e308);  // inf or nan
+      mx = fmax(mx, v);
+      sm += v;
+      code = max(code, low_bit_code(x));
+    }
   }
+  if (nonfinite) *bad = 1;
   red_max[threadIdx.x] = mx;
   red_sum[threadIdx.x] = sm;
   red_code[threadIdx.x] = code;
@@ -251,14 +262,24 @@ __global__ __launch_bounds__(256) void col_stats_kernel(const double *__restrict
   const int r1 = min(k, r0 + rows_per_block);
   double mx = 0.0, sm = 0.0;
   int code = 0;
-  for (int r = r0; r < r1; ++r) {
-    const double x = B[(long)r * ldb + j];
-    const double v = fabs(x);
-    mx = fmax(mx, v);
-    sm += v;
-    code = max(code, low_bit_code(x));
+  bool nonfinite = false;
+  // four loads in flight per thread, the zeros past r1 neutral (as in row_stats_kernel)
+  for (int r = r0; r < r1; r += 4) {
+    double xs[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) xs[q] = load_or_zero(B, (long)(r + q) * ldb + j, r + q < r1);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double x = xs[q];
+      const double v = fabs(x);
+      // per entry: fmax drops a NaN, so the maximum alone would not show one
+      nonfinite |= !(v <= 1.79769313486231570815e308);
+      mx = fmax(mx, v);
+      sm += v;
+      code = max(code, low_bit_code(x));
+    }
   }
-  if (!(mx <= 1.79769313486231570815e308)) *bad = 1;
+  if (nonfinite) *bad = 1;
   // non-negative doubles order like their bit patterns
   atomicMax(&cmax[j], (unsigned long long)__double_as_longlong(mx));
   atomicAdd(&csum[j], sm);
@@ -920,7 +941,7 @@ __global__ __launch_bounds__(256) void bf16x3_rows_kernel(const float *__restric
   for (long c = ((long)blockIdx.x * blockDim.x + threadIdx.x) * 8; c < kp; c += (long)gridDim.x * blockDim.x * 8) {
     uint32_t h[8], mi[8], lo[8];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) split3(c + e < k ? row[c + e] : 0.0f, h[e], mi[e], lo[e]);
+    for (int e = 0; e < 8; ++e) split3(load_or_zero(row, c + e, c + e < k), h[e], mi[e], lo[e]);
     u4v vh, vm, vl;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -940,10 +961,14 @@ __global__ __launch_bounds__(256) void bf16x3_cols_kernel(const float *__restric
   __shared__ float tile[64][65];
   const int k0 = blockIdx.x * 64, j0 = blockIdx.y * 64;
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  for (int r = ty; r < 64; r += 4) {
-    const int kk = k0 + r, jj = j0 + tx;
-    tile[r][tx] = (kk < k && jj < n) ? B[(long)kk * ldb + jj] : 0.0f;
+  float v[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int kk = k0 + ty + 4 * i, jj = j0 + tx;
+    v[i] = load_or_zero(B, (long)kk * ldb + jj, kk < k && jj < n);
   }
+#pragma unroll
+  for (int i = 0; i < 16; ++i) tile[ty + 4 * i][tx] = v[i];
   __syncthreads();
 #pragma unroll
   for (int it = 0; it < 2; ++it) {
@@ -973,7 +998,8 @@ __global__ __launch_bounds__(256) void bf16x3_cols_kernel(const float *__restric
 // one plane row (two per thread) and writes 16 bytes per plane. A's rows are its plane
 // rows, so a thread reads its 8 k straight from the row (16-byte loads where rows are
 // aligned); B's tile passes through LDS as tile[plane row][k], read along B's rows
-// (coalesced). The cut is split3, as in bf16x3_rows_kernel / bf16x3_cols_kernel.
+// (coalesced). Every load of a thread is issued before the first wait. The cut is split3,
+// as in bf16x3_rows_kernel / bf16x3_cols_kernel.
 __global__ __launch_bounds__(256) void bf16x3_cut_kernel(const Bf3Cut *__restrict__ cuts,
                                                          const int *__restrict__ block_cut) {
   __shared__ float tile[64][65];
@@ -984,39 +1010,51 @@ __global__ __launch_bounds__(256) void bf16x3_cut_kernel(const Bf3Cut *__restric
   const int k0 = (lb % c.tiles_k) * 64;
   const int nrows = c.is_b ? c.cols : c.rows;  // plane rows
   const int kk = c.is_b ? c.rows : c.cols;     // k
+  float x2[2][8];
   if (c.is_b) {
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    float v[16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
       // B row k0 + r, columns r0 + tx: plane row tx, k r
       const int r = ty + 4 * i;
       const int sk = k0 + r, sj = r0 + tx;
-      tile[tx][r] = (sk < kk && sj < nrows) ? c.src[(long)sk * c.ld + sj] : 0.0f;
+      v[i] = load_or_zero(c.src, (long)sk * c.ld + sj, sk < kk && sj < nrows);
     }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) tile[tx][ty + 4 * i] = v[i];
     __syncthreads();
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+      const int item = threadIdx.x + it * 256;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) x2[it][e] = tile[item >> 3][(item & 7) * 8 + e];
+    }
+  } else {
+    // both items' loads go out before the first wait
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+      const int item = threadIdx.x + it * 256;
+      const int pr = min(item >> 3, nrows - 1 - r0), kq = (item & 7) * 8;  // rows past the edge: unused
+      const long base = (long)(r0 + pr) * c.ld + k0 + kq;
+      const int krem = kk - (k0 + kq);
+      if (c.vec && krem >= 8) {
+        const float4 v0 = *reinterpret_cast<const float4 *>(c.src + base);
+        const float4 v1 = *reinterpret_cast<const float4 *>(c.src + base + 4);
+        x2[it][0] = v0.x, x2[it][1] = v0.y, x2[it][2] = v0.z, x2[it][3] = v0.w;
+        x2[it][4] = v1.x, x2[it][5] = v1.y, x2[it][6] = v1.z, x2[it][7] = v1.w;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) x2[it][e] = load_or_zero(c.src, base + e, e < krem);
+      }
+    }
   }
 #pragma unroll
   for (int it = 0; it < 2; ++it) {
     const int item = threadIdx.x + it * 256;  // 64 plane rows x 8 groups of 8 k
     const int pr = item >> 3, kq = (item & 7) * 8;
     if (r0 + pr >= nrows) continue;
-    float x[8];
-    if (c.is_b) {
-#pragma unroll
-      for (int e = 0; e < 8; ++e) x[e] = tile[pr][kq + e];
-    } else {
-      const float *row = c.src + (long)(r0 + pr) * c.ld + k0 + kq;
-      const int krem = kk - (k0 + kq);
-      if (c.vec && krem >= 8) {
-        const float4 v0 = *reinterpret_cast<const float4 *>(row);
-        const float4 v1 = *reinterpret_cast<const float4 *>(row + 4);
-        x[0] = v0.x, x[1] = v0.y, x[2] = v0.z, x[3] = v0.w;
-        x[4] = v1.x, x[5] = v1.y, x[6] = v1.z, x[7] = v1.w;
-      } else {
-#pragma unroll
-        for (int e = 0; e < 8; ++e) x[e] = e < krem ? row[e] : 0.0f;
-      }
-    }
+    const float *x = x2[it];
     uint32_t h[8], mi[8], lo[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) split3(x[e], h[e], mi[e], lo[e]);
@@ -1272,7 +1310,8 @@ constexpr int kResColsTK = 128, kResColsTJ = 16;
 static_assert(128 % kResColsTK == 0 && kResColsTK == 4 * 32 && 256 % kResColsTJ == 0, "residue_cols tile");
 
 // A8 row i: residue plane t at bytes [t*kp, t*kp + k), zero to kp. Each thread handles
-// four consecutive k and stores one 32-bit word per plane.
+// four consecutive k and stores one 32-bit word per plane; the next four k are loaded
+// before these are cut.
 template <int L, bool NARROW>
 __global__ __launch_bounds__(256) void residue_rows_kernel(const double *__restrict__ A, long lda, int k, int p,
                                                            long kp, const int *__restrict__ eA, CrtConsts cc,
@@ -1281,14 +1320,19 @@ __global__ __launch_bounds__(256) void residue_rows_kernel(const double *__restr
   const double *row = A + (long)i * lda;
   const int sh = p - eA[i];
   int8_t *out = A8 + (long)i * ld8;
-  for (long l0 = (long)threadIdx.x * 4; l0 < kp; l0 += 1024) {
+  double xs[4];
+  long l0 = (long)threadIdx.x * 4;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) xs[q] = load_or_zero(row, l0 + q, l0 + q < k);
+  for (; l0 < kp; l0 += 1024) {
+    double nx[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) nx[q] = load_or_zero(row, l0 + 1024 + q, l0 + 1024 + q < k);
     int r[4][L];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const long l = l0 + q;
-      const double x = l < k ? rint(ldexp(row[l], sh)) : 0.0;
-      residues_any<L, NARROW>(x, cc, r[q]);
-    }
+    for (int q = 0; q < 4; ++q) residues_any<L, NARROW>(rint(ldexp(xs[q], sh)), cc, r[q]);  // 0 beyond k
+#pragma unroll
+    for (int q = 0; q < 4; ++q) xs[q] = nx[q];
 #pragma unroll
     for (int t = 0; t < L; ++t)
       *reinterpret_cast<uint32_t *>(out + (long)t * kp + l0) = pack4(r[0][t], r[1][t], r[2][t], r[3][t]);
@@ -1308,10 +1352,15 @@ __global__ __launch_bounds__(256) void residue_cols_kernel(const double *__restr
   const int k0 = blockIdx.x * TK;
   const int j0 = blockIdx.y * TJ;
   const int tx = threadIdx.x % TJ, ty = threadIdx.x / TJ;
-  for (int r = ty; r < TK; r += 256 / TJ) {
-    const int kk = k0 + r, jj = j0 + tx;
-    tile[r][tx] = (kk < k && jj < n) ? B[(long)kk * ldb + jj] : 0.0;
+  constexpr int RS = 256 / TJ, NL = TK / RS;  // row step, loads per thread
+  double v[NL];
+#pragma unroll
+  for (int u = 0; u < NL; ++u) {
+    const int kk = k0 + ty + u * RS, jj = j0 + tx;
+    v[u] = load_or_zero(B, (long)kk * ldb + jj, kk < k && jj < n);
   }
+#pragma unroll
+  for (int u = 0; u < NL; ++u) tile[ty + u * RS][tx] = v[u];
   __syncthreads();
   const int kq = (threadIdx.x & 31) * 4;  // four consecutive k
   for (int c = threadIdx.x >> 5; c < TJ; c += 8) {

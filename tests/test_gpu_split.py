@@ -183,13 +183,28 @@ def test_split_zero_rows_and_columns(method):
     assert np.all(np.abs(got - exact) <= 1e-14 * mag), (info.slices, info.int_bits)
 
 
-def test_split_rejects_non_finite():
+@pytest.mark.parametrize("side,bad", [("A", np.inf), ("A", np.nan), ("B", -np.inf), ("B", np.nan)])
+@pytest.mark.parametrize("method", [phpc.SPLIT_CRT, phpc.SPLIT_SLICES])
+def test_split_rejects_non_finite(side, bad, method):
+    """An inf or NaN anywhere in A or B fails the call and leaves C alone (a NaN is checked
+    per entry: the column maximum, an fmax, would drop it). 300 x 700 x 260: the bad entry
+    of A sits at k = 517, the third of a thread's four loads in the row statistics; that of
+    B in the third 256-row block of the column statistics."""
     import torch
-    A = np.ones((8, 8))
-    A[3, 4] = np.inf
-    dC = torch.zeros((8, 8), dtype=torch.float64, device="cuda")
-    with pytest.raises(RuntimeError):
-        phpc.dgemm_split_device(dev(A), dev(np.ones((8, 8))), dC, slices=3)
+    A = np.ones((300, 700))
+    B = np.ones((700, 260))
+    if side == "A":
+        A[201, 517] = bad
+    else:
+        B[517, 201] = bad
+    dC = torch.full((300, 260), 2.0, dtype=torch.float64, device="cuda")
+    try:
+        with pytest.raises(RuntimeError):
+            phpc.dgemm_split_device(dev(A), dev(B), dC, slices=3, method=method)
+    finally:
+        phpc.lib().phpc_split_set_method(phpc.SPLIT_CRT)
+    torch.cuda.synchronize()
+    assert torch.all(dC == 2.0)
 
 
 @pytest.mark.parametrize("dist,method", [(phpc.DIST_UNIFORM, 0), (phpc.DIST_WIDE, 0), (phpc.DIST_UNIFORM, 1),
