@@ -575,11 +575,17 @@ __global__ __launch_bounds__(256) void f32_transpose_kernel(const float *__restr
 #pragma unroll
   for (int u = 0; u < T / RS; ++u) tile[ty + u * RS][tx] = v[u];
   __syncthreads();
+  // 16-byte stores: a thread writes 4 consecutive m of 4 rows k (ldat and m are multiples
+  // of 4, so a group lies wholly below or wholly past ldat)
+  static_assert(T == 64, "store groups assume 64 x 64 tiles");
+  const int q = threadIdx.x & 15;
 #pragma unroll
-  for (int u = 0; u < T / RS; ++u) {
-    const int r = ty + u * RS;
-    const int k = k0 + r, m = m0 + tx;
-    if (k < Kd && m < ldat) At[(long)k * ldat + m] = tile[tx][r];
+  for (int u = 0; u < 4; ++u) {
+    const int r = (threadIdx.x >> 4) + 16 * u;
+    const int k = k0 + r, m = m0 + 4 * q;
+    if (k < Kd && m < ldat)
+      *reinterpret_cast<float4 *>(At + (long)k * ldat + m) =
+          make_float4(tile[4 * q][r], tile[4 * q + 1][r], tile[4 * q + 2][r], tile[4 * q + 3][r]);
   }
 }
 
@@ -603,11 +609,16 @@ __global__ __launch_bounds__(256) void f32_stage_kernel(const F32StageCopy *__re
 #pragma unroll
   for (int i = 0; i < 16; ++i) tile[ty + 4 * i][tx] = v[i];
   __syncthreads();
+  // 16-byte stores of 4 consecutive destination columns (pitch and dc are multiples of 4,
+  // copies 256-byte aligned)
+  const int q = threadIdx.x & 15;
 #pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    const int r = ty + 4 * i;
-    const int dr = d0 + r, dc = e0 + tx;
-    if (dr < c.cols && dc < c.pitch) c.dst[(long)dr * c.pitch + dc] = tile[tx][r];
+  for (int i = 0; i < 4; ++i) {
+    const int r = (threadIdx.x >> 4) + 16 * i;
+    const int dr = d0 + r, dc = e0 + 4 * q;
+    if (dr < c.cols && dc < c.pitch)
+      *reinterpret_cast<float4 *>(c.dst + (long)dr * c.pitch + dc) =
+          make_float4(tile[4 * q][r], tile[4 * q + 1][r], tile[4 * q + 2][r], tile[4 * q + 3][r]);
   }
 }
 
